@@ -1,0 +1,68 @@
+// Shared device/host helpers for the bsmps HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bsmps.h"
+
+// complex128 as two doubles (interleaved re, im in memory, like NumPy's complex128).
+struct cd {
+    double re, im;
+};
+
+// NumPy's complex128 multiply on x86-64 is evaluated in this FMA form; the hafnian and
+// the Gamma epilogue reproduce it bit for bit.  This file is compiled with
+// -ffp-contract=off, so nothing else is fused.
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+    cd r;
+    r.re = __builtin_fma(a.re, b.re, -(a.im * b.im));
+    r.im = __builtin_fma(a.re, b.im, a.im * b.re);
+    return r;
+}
+__device__ __forceinline__ cd cadd(cd a, cd b) {
+    cd r;
+    r.re = a.re + b.re;
+    r.im = a.im + b.im;
+    return r;
+}
+// complex / real as NumPy evaluates it: multiply both parts by the rounded reciprocal.
+__device__ __forceinline__ cd cscale(cd a, double s) {
+    cd r;
+    r.re = a.re * s;
+    r.im = a.im * s;
+    return r;
+}
+__device__ __forceinline__ cd czero() { return cd{0.0, 0.0}; }
+
+__device__ __forceinline__ cd ld_cd(const double* p, long i) {
+    const double2 v = reinterpret_cast<const double2*>(p)[i];
+    return cd{v.x, v.y};
+}
+__device__ __forceinline__ void st_cd(double* p, long i, cd v) {
+    reinterpret_cast<double2*>(p)[i] = make_double2(v.re, v.im);
+}
+
+// Host-side error plumbing shared by every translation unit.
+void bsmps_set_error(const char* fmt, ...);
+int bsmps_check_hip(hipError_t e, const char* what);
+
+struct bsmps_ctx {
+    int device;
+    hipStream_t stream;
+    int own_stream;
+    // device workspace, grown on demand (never on the per-item hot path)
+    void* ws;
+    size_t ws_bytes;
+    // autotuned launch parameters
+    int lane_max_m;   // hafnians of size <= 2*lane_max_m take the one-lane-per-hafnian path
+    int group_lanes;  // lanes per hafnian in the cooperative path (16, 32 or 64)
+};
+
+int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out);
+
+#define BSMPS_TRY(expr)                      \
+    do {                                     \
+        int _st = (expr);                    \
+        if (_st != BSMPS_OK) return _st;     \
+    } while (0)
+#define BSMPS_HIP(expr) BSMPS_TRY(bsmps_check_hip((expr), #expr))

@@ -1,0 +1,105 @@
+// Context, workspace and error reporting for the bsmps C ABI.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "common.h"
+
+static thread_local char g_last_error[512] = "";
+
+void bsmps_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+int bsmps_check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return BSMPS_OK;
+    bsmps_set_error("%s failed: %s", what, hipGetErrorString(e));
+    return BSMPS_ERR_HIP;
+}
+
+int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->ws_bytes) {
+        if (ctx->ws) {
+            BSMPS_HIP(hipStreamSynchronize(ctx->stream));
+            BSMPS_HIP(hipFree(ctx->ws));
+            ctx->ws = nullptr;
+            ctx->ws_bytes = 0;
+        }
+        const size_t want = bytes + bytes / 4;
+        BSMPS_HIP(hipMalloc(&ctx->ws, want));
+        ctx->ws_bytes = want;
+    }
+    *out = ctx->ws;
+    return BSMPS_OK;
+}
+
+extern "C" const char* bsmps_last_error(void) { return g_last_error; }
+
+extern "C" int bsmps_version(void) { return 1; }
+
+extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
+    if (!out) {
+        bsmps_set_error("bsmps_create: null output pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    *out = nullptr;
+    int n = 0;
+    BSMPS_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        bsmps_set_error("bsmps_create: device %d out of range (%d devices)", device, n);
+        return BSMPS_ERR_INVALID;
+    }
+    BSMPS_HIP(hipSetDevice(device));
+    bsmps_ctx* c = new bsmps_ctx();
+    c->device = device;
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    c->lane_max_m = 4;
+    c->group_lanes = 64;
+    if (stream) {
+        c->stream = reinterpret_cast<hipStream_t>(stream);
+        c->own_stream = 0;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return bsmps_check_hip(e, "hipStreamCreateWithFlags");
+        }
+        c->own_stream = 1;
+    }
+    *out = c;
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_destroy(bsmps_ctx* ctx) {
+    if (!ctx) return BSMPS_OK;
+    int st = BSMPS_OK;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = BSMPS_ERR_HIP;
+    if (ctx->ws) hipFree(ctx->ws);
+    if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return st;
+}
+
+extern "C" int bsmps_synchronize(bsmps_ctx* ctx) {
+    if (!ctx) {
+        bsmps_set_error("bsmps_synchronize: null context");
+        return BSMPS_ERR_INVALID;
+    }
+    BSMPS_HIP(hipStreamSynchronize(ctx->stream));
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_set_tuning(bsmps_ctx* ctx, int lane_max_m, int group_lanes) {
+    if (!ctx || lane_max_m < 0 || lane_max_m > 5 ||
+        (group_lanes != 16 && group_lanes != 32 && group_lanes != 64)) {
+        bsmps_set_error("bsmps_set_tuning: lane_max_m must be 0..5 and group_lanes 16, 32 or 64");
+        return BSMPS_ERR_INVALID;
+    }
+    ctx->lane_max_m = lane_max_m;
+    ctx->group_lanes = group_lanes;
+    return BSMPS_OK;
+}

@@ -1,0 +1,447 @@
+// Sampling path (L4): random displacements, site contraction, conditional probabilities
+// and the draw (sampling_cpu.py:30-159; GPU path distributed_sampling.py:62-175).
+//
+//   bsmps_mu_to_alpha      alpha = (sqrtW @ z) -> complex amplitudes            (:37-41)
+//   bsmps_batch_displaces  expm(alpha a^dag - alpha^* a), FP64 spectral form    (:133-148)
+//   bsmps_site_contract    T = (pre * Lambda_prev) @ Gamma, FP64 MFMA GEMM      (:75-80)
+//   bsmps_sample_site      D(alpha) T, |.|^2 Lambda^2, inverse CDF, select      (:81-113)
+//   bsmps_philox_*         counter-based RNG replacing np.random                (:37, 52, 92)
+#include <math.h>
+
+#include <algorithm>
+
+#include "common.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// alpha from normals: mu[i] = sum_k sqrtW[i][k] z[k][s]; alpha = (mu[i] + 1j mu[M+i]) / 2
+// ---------------------------------------------------------------------------------------
+__global__ void mu_to_alpha_kernel(const double* __restrict__ sqrtW, int M, const double* __restrict__ z,
+                                   int64_t n, double* __restrict__ alpha) {
+    const int64_t total = n * M;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = idx / M;
+        const int i = (int)(idx - s * M);
+        double re = 0.0, im = 0.0;
+        const double* wr = sqrtW + (int64_t)i * 2 * M;
+        const double* wi = sqrtW + (int64_t)(M + i) * 2 * M;
+        for (int k = 0; k < 2 * M; ++k) {
+            const double zk = z[(int64_t)k * n + s];
+            re = fma(wr[k], zk, re);
+            im = fma(wi[k], zk, im);
+        }
+        reinterpret_cast<double2*>(alpha)[s * M + i] = make_double2(re * 0.5, im * 0.5);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Displacement operator on the dd-dim truncated space.  With alpha = r e^{i phi},
+// alpha a^dag - alpha^* a = R (r (a^dag - a)) R^-1, R = diag(e^{i n phi}), and
+// a^dag - a = P (-i Q) P^-1 with P = diag(i^n), Q = a + a^dag = W diag(x) W^T (real symmetric
+// tridiagonal).  Hence D[m][k] = e^{i (m - k) theta} sum_q W[m][q] W[k][q] e^{-i r x_q},
+// theta = phi + pi/2 -- the exact exponential of the truncated generator.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void displace_entry_setup(cd alpha, int dd, const double* x, double* cr, double* sr,
+                                                     double& theta) {
+    const double r = hypot(alpha.re, alpha.im);
+    theta = atan2(alpha.im, alpha.re) + 1.5707963267948966;
+    for (int q = 0; q < dd; ++q) {
+        double s, c;
+        sincos(r * x[q], &s, &c);
+        cr[q] = c;
+        sr[q] = -s;  // e^{-i r x_q}
+    }
+}
+
+__global__ void batch_displaces_kernel(int dd, const double* __restrict__ alpha, int64_t n, int64_t astride,
+                                       const double* __restrict__ W, const double* __restrict__ x,
+                                       double* __restrict__ out) {
+    // one sample per block; threads over (m, k)
+    extern __shared__ double sh[];
+    double* cr = sh;
+    double* sr = sh + dd;
+    __shared__ double s_theta;
+    for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const cd a = ld_cd(alpha, s * astride);
+            double th;
+            displace_entry_setup(a, dd, x, cr, sr, th);
+            s_theta = th;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < dd * dd; e += blockDim.x) {
+            const int m = e / dd, k = e - m * dd;
+            double re = 0.0, im = 0.0;
+            for (int q = 0; q < dd; ++q) {
+                const double w = W[m * dd + q] * W[k * dd + q];
+                re = fma(w, cr[q], re);
+                im = fma(w, sr[q], im);
+            }
+            double sp, cp;
+            sincos((double)(m - k) * s_theta, &sp, &cp);
+            st_cd(out, s * dd * dd + e, cd{re * cp - im * sp, re * sp + im * cp});
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Site contraction as a real FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64):
+//   C[s][2c + {0,1}] = sum_{b} A[s][2b + {0,1}] * Bx[2b + {0,1}][2c + {0,1}]
+// with A[s][2b] = Re(pre) lambda, A[s][2b+1] = Im(pre) lambda and Bx the 2x2 real block
+// [[Re g, Im g], [-Im g, Re g]] of each complex Gamma entry g -- so C is the complex product
+// stored interleaved, exactly T's layout.  M = n, N = 2 chi d, K = 2 chi_prev.
+// Workgroup tile 64 x 64 x 16, four waves of 32 x 32 (2 x 2 MFMA blocks), LDS staged.
+// ---------------------------------------------------------------------------------------
+constexpr int TM = 64, TN = 64, TK = 16;
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
+                                                        const double* __restrict__ pre,
+                                                        const double* __restrict__ lam,
+                                                        const double* __restrict__ gamma, double* __restrict__ T) {
+    __shared__ double As[TK][TM + 1];  // As[k][row]
+    __shared__ double Bs[TK][TN + 1];  // Bs[k][col]
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int64_t row0 = (int64_t)blockIdx.y * TM;
+    const int64_t col0 = (int64_t)blockIdx.x * TN;  // real column
+    const int64_t Nreal = 2 * (int64_t)ncols;
+    const int Kreal = 2 * chi_prev;
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+
+    double4_t acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    for (int k0 = 0; k0 < Kreal; k0 += TK) {
+        // A tile: 64 rows x 8 complex b = 512 complex, 2 per thread
+        for (int e = tid; e < TM * (TK / 2); e += 256) {
+            const int r = e / (TK / 2), bb = e % (TK / 2);
+            const int64_t s = row0 + r;
+            const int b = k0 / 2 + bb;
+            double re = 0.0, im = 0.0;
+            if (s < n && b < chi_prev) {
+                const cd p = ld_cd(pre, s * chi_prev + b);
+                const double l = lam[b];
+                re = p.re * l;
+                im = p.im * l;
+            }
+            As[2 * bb][r] = re;
+            As[2 * bb + 1][r] = im;
+        }
+        // B tile: 8 complex b x 32 complex c = 256 complex, 1 per thread, expanded 2 x 2
+        {
+            const int bb = tid / (TN / 2), cc = tid % (TN / 2);
+            const int b = k0 / 2 + bb;
+            const int64_t c = col0 / 2 + cc;
+            double gr = 0.0, gi = 0.0;
+            if (b < chi_prev && c < ncols) {
+                const cd g = ld_cd(gamma, (int64_t)b * ncols + c);
+                gr = g.re;
+                gi = g.im;
+            }
+            Bs[2 * bb][2 * cc] = gr;
+            Bs[2 * bb][2 * cc + 1] = gi;
+            Bs[2 * bb + 1][2 * cc] = -gi;
+            Bs[2 * bb + 1][2 * cc + 1] = gr;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 4) {
+            double a[2], b[2];
+            // operand layout of v_mfma_f64_16x16x4f64: lane holds A[lane%16][lane/16], B[lane/16][lane%16]
+            for (int i = 0; i < 2; ++i) a[i] = As[kk + lane / 16][wm + i * 16 + lane % 16];
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + lane / 16][wn + j * 16 + lane % 16];
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // result layout: lane holds D[4*(lane/16) + r][lane%16], r = 0..3
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int64_t s = row0 + wm + i * 16 + 4 * (lane / 16) + r;
+                const int64_t c = col0 + wn + j * 16 + lane % 16;
+                if (s < n && c < Nreal) T[s * Nreal + c] = acc[i][j][r];
+            }
+}
+
+// ---------------------------------------------------------------------------------------
+// Displacement + probabilities + draw, one workgroup per sample.
+// ---------------------------------------------------------------------------------------
+constexpr int kMaxDD = 64;
+constexpr int kMaxd = 32;
+
+// DMAX / DDMAX bound d and dd at compile time so the per-thread arrays stay in VGPRs.
+template <int DMAX, int DDMAX>
+__global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
+                                                          const double* __restrict__ T, int64_t tstride,
+                                                          const double* __restrict__ alpha, int64_t astride,
+                                                          const double* __restrict__ W, const double* __restrict__ x,
+                                                          const double* __restrict__ lam,
+                                                          const double* __restrict__ uni, int32_t* __restrict__ samples,
+                                                          int64_t sstride, double* __restrict__ pre,
+                                                          double* __restrict__ probs) {
+    __shared__ double cr[DDMAX], sr[DDMAX];
+    __shared__ cd Dm[DDMAX * DMAX];  // D[k][j], j < d
+    __shared__ double red[4][DDMAX];
+    __shared__ int s_pick;
+    __shared__ double s_theta;
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+        if (tid == 0) {
+            double th;
+            displace_entry_setup(ld_cd(alpha, s * astride), dd, x, cr, sr, th);
+            s_theta = th;
+        }
+        __syncthreads();
+        for (int e = tid; e < dd * d; e += blockDim.x) {
+            const int k = e / d, j = e - k * d;
+            double re = 0.0, im = 0.0;
+            for (int q = 0; q < dd; ++q) {
+                const double w = W[k * dd + q] * W[j * dd + q];
+                re = fma(w, cr[q], re);
+                im = fma(w, sr[q], im);
+            }
+            double sp, cp;
+            sincos((double)(k - j) * s_theta, &sp, &cp);
+            Dm[k * DMAX + j] = cd{re * cp - im * sp, re * sp + im * cp};
+        }
+        __syncthreads();
+        const double* Ts = T + 2 * s * tstride;
+        // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda)
+        double pk[DDMAX];
+#pragma unroll
+        for (int k = 0; k < DDMAX; ++k) pk[k] = 0.0;
+        const int a_end = lam ? chi : 1;
+        for (int a = tid; a < a_end; a += blockDim.x) {
+            cd t[DMAX];
+#pragma unroll
+            for (int j = 0; j < DMAX; ++j) t[j] = j < d ? ld_cd(Ts, (int64_t)a * d + j) : czero();
+            const double l2 = lam ? lam[a] * lam[a] : 1.0;
+#pragma unroll
+            for (int k = 0; k < DDMAX; ++k) {
+                if (k < dd) {
+                    double ur = 0.0, ui = 0.0;
+#pragma unroll
+                    for (int j = 0; j < DMAX; ++j) {
+                        if (j < d) {
+                            const cd dk = Dm[k * DMAX + j];
+                            ur = fma(dk.re, t[j].re, fma(-dk.im, t[j].im, ur));
+                            ui = fma(dk.re, t[j].im, fma(dk.im, t[j].re, ui));
+                        }
+                    }
+                    pk[k] = fma(fma(ur, ur, ui * ui), l2, pk[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DDMAX; ++k) {
+            if (k < dd) {
+                double v = pk[k];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+                if (lane == 0) red[wave][k] = v;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double pf[DDMAX];
+            double tot = 0.0;
+#pragma unroll
+            for (int k = 0; k < DDMAX; ++k) {
+                if (k < dd) {
+                    double v = 0.0;
+                    for (int w = 0; w < 4; ++w) v += red[w][k];
+                    pf[k] = v;
+                    tot += v;
+                }
+            }
+            const double u = uni[s];
+            double cdf = 0.0;
+            int cntk = 0;
+#pragma unroll
+            for (int k = 0; k < DDMAX; ++k) {
+                if (k < dd) {
+                    const double pnorm = pf[k] / tot;
+                    cdf += pnorm;
+                    if (u > cdf) ++cntk;
+                    if (probs) probs[s * dd + k] = pnorm;
+                }
+            }
+            s_pick = cntk < dd - 1 ? cntk : dd - 1;
+            samples[s * sstride] = s_pick;
+        }
+        __syncthreads();
+        const int kp = s_pick;
+        if (pre) {
+            for (int a = tid; a < chi; a += blockDim.x) {
+                double ur = 0.0, ui = 0.0;
+                for (int j = 0; j < d; ++j) {
+                    const cd dk = Dm[kp * DMAX + j];
+                    const cd t = ld_cd(Ts, (int64_t)a * d + j);
+                    ur = fma(dk.re, t.re, fma(-dk.im, t.im, ur));
+                    ui = fma(dk.re, t.im, fma(dk.im, t.re, ui));
+                }
+                st_cd(pre, s * chi + a, cd{ur, ui});
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ---------------------------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ inline double u53(uint32_t lo, uint32_t hi) {
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    return (double)(v >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void philox_kernel(uint64_t seed, uint32_t stream0, int n_streams, int64_t first, int64_t n, int normal,
+                              double* out) {
+    const int64_t total = n * n_streams;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = idx / n;
+        const int64_t e = first + (idx - r * n);
+        uint32_t c[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), stream0 + (uint32_t)r, 0u};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u1 = u53(c[0], c[1]);
+        if (!normal) {
+            out[idx] = u1;
+        } else {
+            const double u2 = u53(c[2], c[3]);
+            out[idx] = sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
+        }
+    }
+}
+
+int grid_for(int64_t work, int per_block) {
+    const int64_t b = (work + per_block - 1) / per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 16));
+}
+
+}  // namespace
+
+extern "C" int bsmps_mu_to_alpha(bsmps_ctx* ctx, const double* sqrtW, int M, const double* normals, int64_t n,
+                                 double* alpha) {
+    if (!ctx || M <= 0 || n < 0) {
+        bsmps_set_error("bsmps_mu_to_alpha: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!sqrtW || !normals || !alpha) {
+        bsmps_set_error("bsmps_mu_to_alpha: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(mu_to_alpha_kernel, dim3(grid_for(n * M, 256)), dim3(256), 0, ctx->stream, sqrtW, M,
+                       normals, n, alpha);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_batch_displaces(bsmps_ctx* ctx, int dd, const double* alpha, int64_t n, int64_t alpha_stride,
+                                     const double* eigvec, const double* eigval, double* out) {
+    if (!ctx || dd <= 0 || dd > kMaxDD || n < 0 || alpha_stride < 0) {
+        bsmps_set_error("bsmps_batch_displaces: invalid arguments (dd must be 1..%d)", kMaxDD);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!alpha || !eigvec || !eigval || !out) {
+        bsmps_set_error("bsmps_batch_displaces: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(batch_displaces_kernel, dim3(grid_for(n, 1)), dim3(128), sizeof(double) * 2 * dd, ctx->stream,
+                       dd, alpha, n, alpha_stride, eigvec, eigval, out);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const double* pre,
+                                   const double* lambda_prev, const double* gamma, double* T) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_site_contract: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre || !lambda_prev || !gamma || !T) {
+        bsmps_set_error("bsmps_site_contract: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d;
+    const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
+    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+        bsmps_set_error("bsmps_site_contract: problem too large for one launch");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(site_gemm_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n, chi_prev,
+                       (int)ncols, pre, lambda_prev, gamma, T);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const double* T,
+                                 int64_t T_sample_stride, const double* alpha, int64_t alpha_stride,
+                                 const double* eigvec, const double* eigval, const double* lambda,
+                                 const double* uniforms, int32_t* samples, int64_t samples_stride, double* pre,
+                                 double* probs) {
+    if (!ctx || n < 0 || chi <= 0 || d <= 0 || d > kMaxd || dd < d || dd > kMaxDD || T_sample_stride < 0 ||
+        alpha_stride < 0 || samples_stride < 0) {
+        bsmps_set_error("bsmps_sample_site: invalid arguments (d <= %d, d <= dd <= %d)", kMaxd, kMaxDD);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!T || !alpha || !eigvec || !eigval || !uniforms || !samples) {
+        bsmps_set_error("bsmps_sample_site: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                        \
+    hipLaunchKernelGGL((sample_site_kernel<DM, DDM>), dim3(grid_for(n, 1)), dim3(256), 0, ctx->stream, n, chi, d, dd, \
+                       T, T_sample_stride, alpha, alpha_stride, eigvec, eigval, lambda, uniforms, samples,       \
+                       samples_stride, pre, probs)
+    if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+    else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
+    else BSMPS_SAMPLE_LAUNCH(32, 64);
+#undef BSMPS_SAMPLE_LAUNCH
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_philox(bsmps_ctx* ctx, uint64_t seed, uint32_t stream0, int n_streams, int64_t first, int64_t n,
+                            int normal, double* out) {
+    if (!ctx || n < 0 || n_streams < 0 || first < 0) {
+        bsmps_set_error("bsmps_philox: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0 || n_streams == 0) return BSMPS_OK;
+    if (!out) {
+        bsmps_set_error("bsmps_philox: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(philox_kernel, dim3(grid_for(n * n_streams, 256)), dim3(256), 0, ctx->stream, seed, stream0,
+                       n_streams, first, n, normal, out);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}

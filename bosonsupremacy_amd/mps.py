@@ -1,0 +1,241 @@
+"""Site tensors Gamma_k / Lambda_k (L3) -- drop-in for MPS_cpu.py.
+
+Per site the host computes the site's Gaussian-unitary kernel Sigma and normalisation Z
+(MPS_cpu.py:395, 481-483) and a few small per-label tables (targets, denominators, photon
+classes); every Gamma entry -- target assembly, gather, hafnian and epilogue -- is computed
+on the GPU by ``bsmps_gamma_site``.  The reference functions are available with their
+signatures: ``Sigma_select``, ``hafnian``, ``A_elem`` (GPU-backed), ``get_target`` and
+``push_to_end`` (host integer helpers).
+
+Differences from the reference, by design: complex128 Gamma and float64 Lambda (the
+reference stores complex64/float32, MPS_cpu.py:397-398).
+
+CLI (same flags and files as the reference)::
+
+    python -m bosonsupremacy_amd.mps --d D --chi CHI --dir ROOT/
+"""
+import argparse
+import ctypes
+import time
+
+import numpy as np
+import torch
+from scipy.special import factorial
+
+from . import _lib, gaussian, runtime
+
+ERR_TOL = 1e-10  # MPS_cpu.py:390
+
+
+# ---------------------------------------------------------------------------------------
+# Host integer helpers (MPS_cpu.py:39-47, 314-332)
+# ---------------------------------------------------------------------------------------
+
+def get_target(num):
+    """Occupation rows [B, L] -> 1-based repeated mode indices [B, max photons], zero padded
+    (MPS_cpu.py:314-332)."""
+    num = np.asarray(num)
+    B, L = num.shape
+    counts = num.sum(axis=1).astype(np.int64)
+    width = int(counts.max()) if B else 0
+    target = np.zeros((B, width), dtype=np.int32)
+    if B == 0 or width == 0:
+        return target
+    flat_counts = num.astype(np.int64).ravel()
+    values = np.repeat(np.tile(np.arange(1, L + 1, dtype=np.int32), B), flat_counts)
+    rows = np.repeat(np.arange(B), counts)
+    starts = np.repeat(np.cumsum(counts) - counts, counts)
+    target[rows, np.arange(len(rows)) - starts] = values
+    return target
+
+
+def push_to_end(array):
+    """Right-align the non-zero prefix of each row (MPS_cpu.py:39-47 on get_target output)."""
+    array = np.asarray(array)
+    B, n = array.shape
+    counts = (array != 0).sum(axis=1)
+    out = np.zeros_like(array)
+    rows, cols = np.nonzero(array)
+    out[rows, cols + (n - counts)[rows]] = array[rows, cols]
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# GPU-backed reference functions (MPS_cpu.py:19-37, 117-140, 334-355)
+# ---------------------------------------------------------------------------------------
+
+def _as_complex_dev(a):
+    return runtime.to_device(np.asarray(a, dtype=np.complex128) if not isinstance(a, torch.Tensor) else a,
+                             dtype=torch.complex128)
+
+
+def Sigma_select(Sigma, target):
+    """out[l, r, c] = Sigma[target[l, r], target[l, c]] (direct_mps_kernels.cu:22-39)."""
+    ctx = runtime.context()
+    S = _as_complex_dev(Sigma)
+    T = runtime.to_device(np.asarray(target, dtype=np.int32))
+    B, n = T.shape
+    out = torch.empty((B, n, n), dtype=torch.complex128, device=ctx.device)
+    ctx.call("bsmps_sigma_select", runtime.ptr(S), S.shape[0], runtime.ptr(T), B, n, runtime.ptr(out))
+    return runtime.to_host(out)
+
+
+def hafnian(A):
+    """Batched hafnian of A [B, n, n] (MPS_cpu.py:117-187), complex128."""
+    ctx = runtime.context()
+    Ad = _as_complex_dev(A)
+    B, n, n2 = Ad.shape
+    if n != n2:
+        raise ValueError("Matrix must be square")
+    out = torch.empty((B,), dtype=torch.complex128, device=ctx.device)
+    ctx.call("bsmps_hafnian_dense", runtime.ptr(Ad), B, n, runtime.ptr(out))
+    return runtime.to_host(out)
+
+
+def A_elem(Sigma, target, denominator, max_memory_in_gb=0.5):
+    """haf(Sigma[t][:, t]) / denominator for each target row (MPS_cpu.py:334-355).
+
+    Returns (values, haf_time, sigma_time) like the reference; the gather is fused into the
+    hafnian kernel, so sigma_time is 0.  ``max_memory_in_gb`` is accepted for signature
+    compatibility; the GPU path never materialises the gathered matrices."""
+    ctx = runtime.context()
+    S = _as_complex_dev(Sigma)
+    T = runtime.to_device(np.ascontiguousarray(target, dtype=np.int32))
+    den = runtime.to_device(np.ascontiguousarray(np.broadcast_to(denominator, (T.shape[0],)), dtype=np.float64))
+    out = torch.empty((T.shape[0],), dtype=torch.complex128, device=ctx.device)
+    t0 = time.time()
+    ctx.call("bsmps_hafnian_targets", runtime.ptr(S), S.shape[0], runtime.ptr(T), T.shape[0], T.shape[1],
+             runtime.ptr(den), runtime.ptr(out))
+    res = runtime.to_host(out)
+    return res, time.time() - t0, 0.0
+
+
+# ---------------------------------------------------------------------------------------
+# Site tensors
+# ---------------------------------------------------------------------------------------
+
+class _Labels:
+    """Per-label tables of one side of a site: Sigma targets, denominators, photon classes."""
+
+    def __init__(self, num, offset):
+        num = np.asarray(num)
+        num = num.reshape(num.shape[0], -1)
+        self.count = num.sum(axis=1).astype(np.int32)
+        tg = get_target(num)
+        if offset:
+            tg = np.where(tg != 0, tg + offset, 0).astype(np.int32)
+        self.targets = tg
+        self.den = np.sqrt(np.prod(np.array(factorial(num)), axis=1)).astype(np.float64)
+        self.perm = np.argsort(self.count, kind='stable').astype(np.int32)
+        self.class_count = np.bincount(self.count).astype(np.int32) if len(self.count) else np.zeros(1, np.int32)
+
+    @classmethod
+    def empty(cls):
+        self = cls.__new__(cls)
+        self.count = np.zeros(1, np.int32)
+        self.targets = np.zeros((1, 0), np.int32)
+        self.den = np.ones(1)
+        self.perm = np.zeros(1, np.int32)
+        self.class_count = np.ones(1, np.int32)
+        return self
+
+
+def _dev_i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    if a.size == 0:
+        a = np.zeros(1, np.int32)
+    return runtime.to_device(a)
+
+
+def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL):
+    """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
+
+    Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site."""
+    ctx = runtime.context()
+    dev = ctx.device
+    Sigma, Z = gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
+    Mp = Sigma.shape[0] // 2
+    Lambda = np.zeros(chi)
+    if k == M - 1:
+        kind = _lib.SITE_LAST
+        left = _Labels.empty()
+        lam = np.ones(1)
+    else:
+        kind = _lib.SITE_FIRST if k == 0 else _lib.SITE_MIDDLE
+        res = np.asarray(kron_res[k], dtype=np.float64)
+        keep = res > err_tol
+        left = _Labels(np.asarray(kron_num[k]).reshape(len(res), -1)[keep], 0)
+        lam = np.sqrt(res[keep])
+        Lambda[:len(lam)] = lam
+    if k == 0:
+        right = _Labels.empty()
+    else:
+        num_pre = np.asarray(kron_num[k - 1])
+        right = _Labels(num_pre.reshape(num_pre.shape[0], -1), Mp - 1)
+    sqrt_fact = np.array([np.sqrt(factorial(j)) for j in np.arange(d)], dtype=np.float64)
+
+    sig_d = runtime.to_device(Sigma)
+    sf_d = runtime.to_device(sqrt_fact)
+    lt_d, lp_d = _dev_i32(left.targets), _dev_i32(left.perm)
+    rt_d, rp_d = _dev_i32(right.targets), _dev_i32(right.perm)
+    lden_d, rden_d, lam_d = runtime.to_device(left.den), runtime.to_device(right.den), runtime.to_device(lam)
+    lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
+    rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
+    args = _lib.GammaSiteArgs(
+        kind=kind, d=d, chi=chi, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
+        sqrt_fact=sf_d.data_ptr(),
+        n_left=len(left.count), w_left=left.targets.shape[1], left_targets=lt_d.data_ptr(),
+        left_den=lden_d.data_ptr(), lambda_=lam_d.data_ptr(), left_perm=lp_d.data_ptr(),
+        left_class_count_host=lcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_left_class=len(lcc),
+        n_right=len(right.count), w_right=right.targets.shape[1], right_targets=rt_d.data_ptr(),
+        right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
+        right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
+    gamma = torch.zeros((chi, chi, d), dtype=torch.complex128, device=dev)
+    ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
+    return gamma, Lambda
+
+
+def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5):
+    """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
+    outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them."""
+    from . import kron as kron_mod
+    M = len(sq_cov) // 2
+    if kron is None:
+        kron = kron_mod.kron_all(sq_cov, d, chi, max_dim)
+    kres, knum, kS = kron
+    S_full = gaussian.williamson(sq_cov)[1]
+    Gammas, Lambdas = [], []
+    for k in range(M):
+        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full)
+        Gammas.append(G)
+        Lambdas.append(L)
+    return Gammas, Lambdas, kron
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description="MPS site tensors (drop-in for MPS_cpu.py)")
+    parser.add_argument('--d', type=int, required=True)
+    parser.add_argument('--chi', type=int, required=True)
+    parser.add_argument('--dir', type=str, required=True)
+    args = parser.parse_args(argv)
+    d, chi = args.d, args.chi
+    path = args.dir + f"d_{d}_chi_{chi}/"
+    sq_cov = np.load(args.dir + "sq_cov.npy")
+    cov = np.load(args.dir + "cov.npy")
+    M = len(cov) // 2
+    kres = [np.load(path + f'res_{k}.npy').astype(np.float64) for k in range(M - 1)]
+    knum = [np.load(path + f'num_{k}.npy') for k in range(M - 1)]
+    kS = [np.load(path + f'S_{k}.npy') for k in range(M - 1)]
+    S_full = gaussian.williamson(sq_cov)[1]
+    for k in range(M):
+        t0 = time.time()
+        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full)
+        Gh = runtime.to_host(G)
+        print('mode: ', k, 'Total {:.3f}'.format(time.time() - t0))
+        np.save(path + f"Gamma_{k}.npy", Gh)
+        if k < M - 1:
+            np.save(path + f"Lambda_{k}.npy", L)
+
+
+if __name__ == "__main__":
+    main()

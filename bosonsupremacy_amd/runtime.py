@@ -1,0 +1,87 @@
+"""Device runtime: one bsmps context per GPU, enqueued on torch's current stream.
+
+PyTorch is used only as device-memory and stream plumbing: buffers are torch tensors, and
+their data pointers go to the HIP kernels through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_contexts = {}
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("bosonsupremacy_amd needs a ROCm GPU; none is visible")
+
+
+class Context:
+    """Owns a bsmps_ctx bound to ``device`` and the torch stream current at creation."""
+
+    def __init__(self, device=0):
+        _require_gpu()
+        lib = _lib.load()
+        self.device = torch.device("cuda", device)
+        self.stream = torch.cuda.current_stream(self.device)
+        handle = ctypes.c_void_p()
+        _lib.check(lib.bsmps_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(handle)))
+        self.handle = handle
+        self.lib = lib
+
+    def call(self, name, *args):
+        return _lib.check(getattr(self.lib, name)(self.handle, *args))
+
+    def synchronize(self):
+        self.call("bsmps_synchronize")
+
+    def set_tuning(self, lane_max_m, group_lanes):
+        self.call("bsmps_set_tuning", lane_max_m, group_lanes)
+
+    def close(self):
+        if self.handle:
+            _lib.check(self.lib.bsmps_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def context(device=0):
+    """The process-wide context of ``device`` (created on first use)."""
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _contexts[device] = ctx
+    return ctx
+
+
+def ptr(t):
+    """Device pointer of a contiguous tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def to_device(a, dtype=None, device=None):
+    """numpy (or tensor) -> contiguous device tensor; complex128 stays complex128."""
+    device = device or context().device
+    if isinstance(a, torch.Tensor):
+        t = a.to(device=device, dtype=dtype) if dtype is not None else a.to(device=device)
+        return t.contiguous()
+    arr = np.ascontiguousarray(a)
+    t = torch.from_numpy(arr)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device).contiguous()
+
+
+def to_host(t):
+    return t.detach().cpu().numpy()

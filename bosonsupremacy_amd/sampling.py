@@ -1,0 +1,184 @@
+"""Photon-number sampling (L4) -- drop-in for sampling_cpu.py.
+
+The MPS is walked site by site with all samples of a batch advanced together (site-major):
+for site i the carried vectors go through the FP64 MFMA contraction with Gamma_i
+(``bsmps_site_contract``), then the per-sample displacement, conditional probabilities and
+draw (``bsmps_sample_site``).  Random numbers come from counter-based Philox streams on the
+device, so a sample's randomness depends only on (seed, sample index):
+
+* normals  z[k][s]  -- stream k (k = 0..2M-1), element s;
+* uniforms u[i][s]  -- stream UNIFORM_STREAM + i (site i), element s.
+
+Deviation from the reference, by design: a draw is clamped to dd-1 when rounding leaves the
+last cumulative probability below the uniform (sampling_cpu.py:103-104 would return dd).
+
+CLI (same flags and files as the reference)::
+
+    python -m bosonsupremacy_amd.sampling --N 1000 --n 100 --iter 1 --d 4 --dd 10 --chi 64 --dir ROOT/
+"""
+import argparse
+
+import numpy as np
+import torch
+
+from . import runtime
+
+UNIFORM_STREAM = 0x10000
+HBAR = 2
+
+
+def destroy(N):
+    """Truncated annihilation operator (sampling_cpu.py:120-122), complex128."""
+    return np.diag(np.sqrt(np.arange(1, N, dtype=np.complex128)), 1)
+
+
+def displacement_basis(dd):
+    """Eigen-decomposition of the truncated a + a^dag: (W [dd, dd], x [dd])."""
+    a = np.diag(np.sqrt(np.arange(1, dd, dtype=np.float64)), 1)
+    x, W = np.linalg.eigh(a + a.T)
+    return np.ascontiguousarray(W), np.ascontiguousarray(x)
+
+
+def batch_mu_to_alpha(mu, hbar=HBAR):
+    """sampling_cpu.py:156-159 (host helper, for API parity)."""
+    M = mu.shape[1] // 2
+    return (mu[:, :M] + 1j * mu[:, M:]) / np.sqrt(2 * hbar)
+
+
+def thermal_sqrtW(cov, sq_cov):
+    """Factor L (L L^T = W) of the classical noise W = cov - sq_cov: the regularised Cholesky
+    of sampling_cpu.py:168-170, or, when W is singular to working precision and that
+    Cholesky fails, V diag(sqrt(max(lambda, 0))) from the eigendecomposition of W."""
+    W = cov - sq_cov
+    reg = W + 1.000001 * np.eye(len(W)) * np.abs(np.min(np.linalg.eigvalsh(W)))
+    try:
+        return np.linalg.cholesky(reg)
+    except np.linalg.LinAlgError:
+        lam, V = np.linalg.eigh(0.5 * (W + W.T))
+        return V * np.sqrt(np.maximum(lam, 0.0))[np.newaxis, :]
+
+
+def batch_displaces(N, alphas):
+    """expm(alpha a^dag - alpha^* a) for alphas [B, M] -> [B, M, N, N] (sampling_cpu.py:133-148),
+    computed on the GPU in FP64."""
+    ctx = runtime.context()
+    al = np.ascontiguousarray(alphas, dtype=np.complex128)
+    B, M = al.shape
+    W, x = displacement_basis(N)
+    a_d = runtime.to_device(al)
+    W_d, x_d = runtime.to_device(W), runtime.to_device(x)
+    out = torch.empty((B * M, N, N), dtype=torch.complex128, device=ctx.device)
+    ctx.call("bsmps_batch_displaces", N, runtime.ptr(a_d), B * M, 1, runtime.ptr(W_d), runtime.ptr(x_d),
+             runtime.ptr(out))
+    return runtime.to_host(out).reshape(B, M, N, N)
+
+
+def philox(n_streams, n, seed, stream0, first=0, normal=False):
+    """Device array [n_streams, n]: elements first..first+n-1 of Philox streams stream0.."""
+    ctx = runtime.context()
+    out = torch.empty((n_streams, n), dtype=torch.float64, device=ctx.device)
+    ctx.call("bsmps_philox", int(seed), int(stream0), int(n_streams), int(first), int(n), int(bool(normal)),
+             runtime.ptr(out))
+    return out
+
+
+def random_inputs(M, n, seed, offset=0):
+    """Device normals [2M, n] and uniforms [M, n] for samples offset..offset+n-1."""
+    normals = philox(2 * M, n, seed, 0, offset, normal=True)
+    uniforms = philox(M, n, seed, UNIFORM_STREAM, offset)
+    return normals, uniforms
+
+
+class Sampler:
+    """Samples photon numbers from an MPS held as device tensors.
+
+    gammas : list of M device tensors [chi, chi, d] complex128
+    lambdas: list of M float64 arrays [chi] (the last one unused)
+    sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
+    """
+
+    def __init__(self, gammas, lambdas, sqrtW, dd):
+        self.ctx = runtime.context()
+        dev = self.ctx.device
+        self.M = len(gammas)
+        self.chi, _, self.d = gammas[0].shape
+        self.dd = dd
+        if dd < self.d:
+            raise ValueError("dd must be >= d")
+        self.gammas = [g.contiguous() for g in gammas]
+        self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
+        self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
+        W, x = displacement_basis(dd)
+        self.W = runtime.to_device(W)
+        self.x = runtime.to_device(x)
+        self.dev = dev
+
+    def run(self, normals, uniforms, return_probs=False):
+        """normals [2M, n], uniforms [M, n] (device float64) -> samples [n, M] int32 (device),
+        and optionally the conditional distributions [M, n, dd]."""
+        ctx, M, chi, d, dd = self.ctx, self.M, self.chi, self.d, self.dd
+        n = normals.shape[1]
+        normals = normals.contiguous()
+        uniforms = uniforms.contiguous()
+        alpha = torch.empty((n, M), dtype=torch.complex128, device=self.dev)
+        ctx.call("bsmps_mu_to_alpha", runtime.ptr(self.sqrtW), M, runtime.ptr(normals), n, runtime.ptr(alpha))
+        samples = torch.empty((n, M), dtype=torch.int32, device=self.dev)
+        pre = torch.empty((n, chi), dtype=torch.complex128, device=self.dev)
+        T = torch.empty((n, chi, d), dtype=torch.complex128, device=self.dev)
+        probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
+        import ctypes
+        for i in range(M):
+            last = i == M - 1
+            if i == 0:
+                src, stride = self.gammas[0], 0  # Gamma_0[0, a, j] is the site-0 tensor for every sample
+            else:
+                ctx.call("bsmps_site_contract", n, chi, chi, d, runtime.ptr(pre), runtime.ptr(self.lambdas[i - 1]),
+                         runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                src, stride = T, chi * d
+            a_ptr = ctypes.c_void_p(alpha.data_ptr() + 16 * i)
+            s_ptr = ctypes.c_void_p(samples.data_ptr() + 4 * i)
+            p_ptr = runtime.ptr(probs[i]) if probs is not None else None
+            ctx.call("bsmps_sample_site", n, chi, d, dd, runtime.ptr(src), stride, a_ptr, M, runtime.ptr(self.W),
+                     runtime.ptr(self.x), None if last else runtime.ptr(self.lambdas[i]),
+                     runtime.ptr(uniforms[i]), s_ptr, M, None if last else runtime.ptr(pre), p_ptr)
+        return (samples, probs) if return_probs else samples
+
+
+def sampling(gammas, lambdas, sqrtW, dd, n, seed=0, offset=0):
+    """Draw n samples (host int64 array [n, M]) with the device Philox streams."""
+    s = Sampler(gammas, lambdas, sqrtW, dd)
+    normals, uniforms = random_inputs(s.M, n, seed, offset)
+    return runtime.to_host(s.run(normals, uniforms)).astype(np.int64)
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description="Photon-number sampling (drop-in for sampling_cpu.py)")
+    parser.add_argument('--N', type=int, required=True, help='total number of samples')
+    parser.add_argument('--n', type=int, required=True, help='samples per batch')
+    parser.add_argument('--iter', type=int, default=1)
+    parser.add_argument('--d', type=int, required=True)
+    parser.add_argument('--dd', type=int, required=True)
+    parser.add_argument('--chi', type=int, required=True)
+    parser.add_argument('--dir', type=str, required=True)
+    parser.add_argument('--seed', type=int, default=0)
+    args = parser.parse_args(argv)
+    path = args.dir + f'd_{args.d}_chi_{args.chi}/'
+    sq_cov = np.load(args.dir + "sq_cov.npy")
+    cov = np.load(args.dir + "cov.npy")
+    sqrtW = thermal_sqrtW(cov, sq_cov)
+    M = sqrtW.shape[0] // 2
+    dev = runtime.context().device
+    gammas = [torch.from_numpy(np.load(path + f'Gamma_{i}.npy').astype(np.complex128)).to(dev) for i in range(M)]
+    lambdas = [np.load(path + f"Lambda_{i}.npy").astype(np.float64) for i in range(M - 1)] + [np.zeros(args.chi)]
+    sampler = Sampler(gammas, lambdas, sqrtW, args.dd)
+    for it in range(args.iter):
+        chunks = []
+        for b0 in range(0, args.N, args.n):
+            nb = min(args.N, b0 + args.n) - b0
+            normals, uniforms = random_inputs(M, nb, args.seed + it, b0)
+            chunks.append(runtime.to_host(sampler.run(normals, uniforms)).astype(np.int64))
+            np.save(args.dir + f"samples_{it}.npy", np.concatenate(chunks, axis=0))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,180 @@
+/* bsmps -- MI355X (gfx950) MPS construction and sampling for lossy Gaussian boson sampling.
+ *
+ * C ABI.  Plain pointers and sizes; no framework types.  Conventions:
+ *   - every entry point returns BSMPS_OK (0) or an error code; bsmps_last_error() gives a
+ *     thread-local message for the last failure;
+ *   - array arguments are caller-owned DEVICE memory unless the name ends in _host;
+ *   - complex arrays are complex128, interleaved (re, im), row-major, like NumPy's;
+ *   - work is enqueued on the context's stream and is asynchronous unless stated;
+ *     bsmps_synchronize() waits for it;
+ *   - a context is used from one host thread at a time; contexts are independent.
+ * Sizes are checked on the host before any launch; a bad size returns BSMPS_ERR_INVALID
+ * and launches nothing.
+ *
+ * Reference interfaces each entry point replaces are cited as file:line of the reference
+ * repository (sss441803/BosonSupremacy).
+ */
+#ifndef BSMPS_H
+#define BSMPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BSMPS_OK = 0,
+    BSMPS_ERR_INVALID = 1,     /* bad argument (size, null pointer, unsupported shape) */
+    BSMPS_ERR_HIP = 2,         /* a HIP runtime call failed */
+    BSMPS_ERR_UNSUPPORTED = 3, /* valid request outside what this build implements */
+};
+
+/* Largest hafnian matrix size (rows) the hafnian entry points accept. */
+#define BSMPS_MAX_HAFNIAN_SIZE 24
+
+typedef struct bsmps_ctx bsmps_ctx;
+
+/* ---- context --------------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  `stream` is a hipStream_t to enqueue on, or
+ * NULL for a stream owned by the context.  (The reference has no context: it runs on
+ * CuPy's current device and default stream, MPS_utils.py:15-19.) */
+int bsmps_create(int device, void *stream, bsmps_ctx **out);
+int bsmps_destroy(bsmps_ctx *ctx);
+int bsmps_synchronize(bsmps_ctx *ctx);
+const char *bsmps_last_error(void);
+int bsmps_version(void);
+/* Launch tuning: hafnians of size <= 2*lane_max_m use one lane per matrix (lane_max_m in
+ * 0..5); larger ones use `group_lanes` (16, 32 or 64) cooperating lanes per matrix. */
+int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
+
+/* ---- hafnians (L3) --------------------------------------------------------------------- */
+
+/* Sigma_select, MPS_cpu.py:19-37 / direct_mps_kernels.cu:22-39 (sigma_select_cdouble):
+ * out[l, r, c] = sigma[targets[l, r] * ld + targets[l, c]].
+ * sigma [ld][ld], targets int32 [n_batch][n_select], out [n_batch][n_select][n_select]. */
+int bsmps_sigma_select(bsmps_ctx *ctx, const double *sigma, int ld, const int32_t *targets,
+                       int64_t n_batch, int n_select, double *out);
+
+/* hafnian, MPS_cpu.py:117-187 (dispatch + recursive_hafnian + solve): out[b] = haf(A[b]) for
+ * A [n_batch][n][n].  Bit-exact to the reference's complex128 evaluation order.  Odd n gives
+ * zeros, n = 0 gives ones (MPS_cpu.py:122-126). */
+int bsmps_hafnian_dense(bsmps_ctx *ctx, const double *A, int64_t n_batch, int n, double *out);
+
+/* A_elem, MPS_cpu.py:334-355 (MPS_utils.py:273-297): out[b] = haf(Sigma[t_b][:, t_b]) / den[b]
+ * with the gather fused (the n x n matrices are never materialised).  den may be NULL
+ * (no division).  Division is NumPy's complex/real: both parts times 1/den. */
+int bsmps_hafnian_targets(bsmps_ctx *ctx, const double *sigma, int ld, const int32_t *targets,
+                          int64_t n_batch, int n_select, const double *den, double *out);
+
+/* One site tensor Gamma_k, the j / photon-total / chunk loops of MPS_cpu.py:402-520
+ * (first site :402-428, last site :430-459, middle sites :461-520; GPU path
+ * distributed_MPS.py:160-258).  Every entry Gamma[b, a, j] with j + |a| + |b| even is
+ * computed from its labels on the device:
+ *   target = [0]*j ++ left_targets[a] ++ right_targets[b]      (Sigma indices)
+ *   Gamma  = ((haf * (1/den)) * (1/Z)) * (1/lambda[a]),  den = (sqrt(j!) * left_den[a]) * right_den[b]
+ * and, for kind != FIRST, the vacuum entry (j = |a| = |b| = 0) is (1/Z) / lambda[a]
+ * (MPS_cpu.py:449-451, 494-496).  gamma must be zeroed by the caller; odd-size entries are
+ * left untouched (the reference's hafnian is 0 there). */
+enum { BSMPS_SITE_FIRST = 0, BSMPS_SITE_MIDDLE = 1, BSMPS_SITE_LAST = 2 };
+typedef struct {
+    int kind;                     /* BSMPS_SITE_* */
+    int d;                        /* local cutoff: j = 0..d-1 */
+    int chi;                      /* Gamma is [chi][chi][d] */
+    int ld;                       /* Sigma is [ld][ld], ld = 2 M' */
+    const double *sigma;          /* device complex128 [ld][ld] */
+    double z;                     /* Z = sqrt(prod cosh(sq)), MPS_cpu.py:483 */
+    double inv_z;                 /* 1.0 / Z */
+    const double *sqrt_fact;      /* device [d]: sqrt(j!) */
+    /* left labels a (Schmidt vectors of cut k; a single empty label for the last site) */
+    int n_left, w_left;           /* count, row width of left_targets */
+    const int32_t *left_targets;  /* device [n_left][w_left], Sigma indices in 1..M'-1 */
+    const double *left_den;       /* device [n_left]: sqrt(prod num_a!) (1.0 for last site) */
+    const double *lambda;         /* device [n_left]: Lambda_k[a] (1.0 for last site) */
+    const int32_t *left_perm;     /* device [n_left]: labels ordered by photon count */
+    const int32_t *left_class_count_host; /* host [n_left_class]: labels with p photons */
+    int n_left_class;
+    /* right labels b (Schmidt vectors of cut k-1; a single empty label for the first site) */
+    int n_right, w_right;
+    const int32_t *right_targets; /* device [n_right][w_right], Sigma indices in M'..2M'-1 */
+    const double *right_den;      /* device [n_right] (1.0 for the first site) */
+    const int32_t *right_perm;    /* device [n_right] */
+    const int32_t *right_class_count_host; /* host [n_right_class] */
+    int n_right_class;
+} bsmps_gamma_site_args;
+
+int bsmps_gamma_site(bsmps_ctx *ctx, const bsmps_gamma_site_args *args, double *gamma);
+
+/* ---- Schmidt spectra (L2) -------------------------------------------------------------- */
+
+/* The pruned Kronecker loop of get_cumsum_kron, kron_cpu.py:106-125 (GPU path
+ * distributed_kron.py:105-137), for n_cuts independent cuts in one launch.  Cut c has
+ * n_modes_host[c] >= 1 modes with per-mode thermal distributions tp[c][i][0..d-1]
+ * (thermal_photons, kron_cpu.py:92-93, evaluated on the host).  Per step the candidates
+ * res[a]*tp[i][n] above err_tol are reduced to the max_dim largest; the chi largest of the
+ * last step are returned in descending order with their occupation labels.
+ *   tp   device double [n_cuts][max_modes][d]
+ *   res  device double [n_cuts][chi]            (descending; unused tail zero)
+ *   num  device int8   [n_cuts][chi][max_modes] (label of each kept weight)
+ *   len  device int32  [n_cuts]                 (number of kept weights, <= chi)
+ * Exact ties are ordered by (mode-i photon number, then earlier rank); the reference's
+ * argsort leaves their order unspecified. */
+int bsmps_kron_cuts(bsmps_ctx *ctx, int n_cuts, const int32_t *n_modes_host, int max_modes, int d,
+                    const double *tp, int64_t max_dim, int chi, double err_tol, double *res,
+                    int8_t *num, int32_t *len);
+
+/* ---- sampling (L4) --------------------------------------------------------------------- */
+
+/* alpha[s][i] = (mu[s][i] + 1j mu[s][M+i]) / 2 with mu = sqrtW @ normals[:, s]
+ * (sampling_cpu.py:37-41 and batch_mu_to_alpha :156-159, hbar = 2).
+ * sqrtW device [2M][2M], normals device [2M][n], alpha device complex128 [n][M]. */
+int bsmps_mu_to_alpha(bsmps_ctx *ctx, const double *sqrtW, int M, const double *normals,
+                      int64_t n, double *alpha);
+
+/* batch_displaces, sampling_cpu.py:133-148: D[s] = expm(alpha_s a^dag - conj(alpha_s) a) of the
+ * dd x dd truncated ladder operators, in FP64 via the exact spectral form
+ * D[m][k] = e^{i(m-k)theta} sum_q W[m][q] W[k][q] e^{-i|alpha| x_q}, theta = arg(alpha)+pi/2,
+ * where W, x are the eigenvectors/eigenvalues of a + a^dag (dd x dd, computed once on the
+ * host).  alpha complex128 with element stride alpha_stride (in complex elements);
+ * eigvec [dd][dd] (row m, column q), eigval [dd]; out complex128 [n][dd][dd]. */
+int bsmps_batch_displaces(bsmps_ctx *ctx, int dd, const double *alpha, int64_t n,
+                          int64_t alpha_stride, const double *eigvec, const double *eigval,
+                          double *out);
+
+/* Site contraction, sampling_cpu.py:75-80 (distributed_sampling.py:116-117):
+ * T[s][a][j] = sum_b (pre[s][b] * lambda_prev[b]) * gamma[b][a][j],  j < d,
+ * as an FP64 MFMA GEMM.  pre [n][chi_prev], gamma [chi_prev][chi][d], T [n][chi][d]. */
+int bsmps_site_contract(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d,
+                        const double *pre, const double *lambda_prev, const double *gamma,
+                        double *T);
+
+/* Displacement, conditional probabilities and draw of one site, sampling_cpu.py:81-113
+ * (first site :55-72): per sample s,
+ *   U[a][k]  = sum_{j<d} D_s[k][j] T[s][a][j]         (D_s from alpha_s as above)
+ *   p[k]     = sum_a |U[a][k]|^2 lambda[a]^2, or |U[0][k]|^2 when lambda is NULL (last site)
+ *   n_s      = #{k : u_s > cumsum(p / sum p)[k]}, clamped to dd-1
+ *   pre[s][a] = U[a][n_s]
+ * T is read at T + s*T_sample_stride (complex elements; stride 0 broadcasts one [chi][d]
+ * tensor, as the first site does).  probs (may be NULL) receives p / sum p [n][dd]. */
+int bsmps_sample_site(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, const double *T,
+                      int64_t T_sample_stride, const double *alpha, int64_t alpha_stride,
+                      const double *eigvec, const double *eigval, const double *lambda,
+                      const double *uniforms, int32_t *samples, int64_t samples_stride,
+                      double *pre, double *probs);
+
+/* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
+ * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:
+ * counter (e_lo, e_hi, stream, 0) with e = first+i, key (seed_lo, seed_hi).  A uniform is
+ * (x >> 11) * 2^-53 of the first two output words (x = w1 << 32 | w0); a normal (normal != 0)
+ * is sqrt(-2 ln(1 - u1)) cos(2 pi u2) with u2 from words 2, 3 (Box-Muller, cosine branch).
+ * Replaces np.random.normal / np.random.rand, sampling_cpu.py:37, 52, 92.
+ * out device double [n_streams][n]. */
+int bsmps_philox(bsmps_ctx *ctx, uint64_t seed, uint32_t stream0, int n_streams, int64_t first, int64_t n,
+                 int normal, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSMPS_H */

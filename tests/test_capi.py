@@ -1,0 +1,41 @@
+"""The C ABI library loads and exports every entry point include/bsmps.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from bosonsupremacy_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "bsmps.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bsmps_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_all_symbols():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libbsmps.so not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_error_paths_without_device():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libbsmps.so not built")
+    lib = _lib.load()
+    assert lib.bsmps_version() >= 1
+    assert lib.bsmps_set_tuning(None, 4, 64) == _lib.BSMPS_ERR_INVALID
+    assert b"set_tuning" in lib.bsmps_last_error()
+    assert lib.bsmps_synchronize(None) == _lib.BSMPS_ERR_INVALID
+    assert lib.bsmps_create(0, None, None) == _lib.BSMPS_ERR_INVALID
+    with pytest.raises(_lib.BsmpsError):
+        _lib.check(lib.bsmps_synchronize(None))

@@ -1,0 +1,104 @@
+"""GPU sampling path vs the oracle: displacements, contraction, probabilities, draws, RNG."""
+import numpy as np
+import pytest
+import torch
+from scipy.linalg import expm
+
+from conftest import same_bits
+from oracle import gbs_oracle as O, philox
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_displaces_matches_expm():
+    from bosonsupremacy_amd import sampling
+    rng = np.random.default_rng(0)
+    for dd in (4, 10, 20):
+        al = (rng.normal(size=(40, 3)) + 1j * rng.normal(size=(40, 3))) * 0.8
+        got = sampling.batch_displaces(dd, al)
+        want = O.batch_displaces(dd, al)
+        assert np.abs(got - want).max() < 1e-12, dd
+
+
+def test_site_contract_matches_matmul():
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(1)
+    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4), (5, 100, 7, 6)):
+        pre = rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))
+        lam = rng.uniform(0.1, 1, size=chi_p)
+        G = rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))
+        want = ((pre * lam) @ G.reshape(chi_p, chi * d)).reshape(n, chi, d)
+        pre_d, lam_d, G_d = runtime.to_device(pre), runtime.to_device(lam), runtime.to_device(G)
+        T = torch.empty((n, chi, d), dtype=torch.complex128, device=ctx.device)
+        ctx.call("bsmps_site_contract", n, chi_p, chi, d, runtime.ptr(pre_d), runtime.ptr(lam_d),
+                 runtime.ptr(G_d), runtime.ptr(T))
+        got = runtime.to_host(T)
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max(), (n, chi_p, chi, d)
+
+
+def test_philox_matches_host():
+    from bosonsupremacy_amd import sampling, runtime
+    for seed, stream0, first, n in ((0, 0, 0, 1000), (12345678901, 7, 10 ** 9, 257)):
+        u = runtime.to_host(sampling.philox(3, n, seed, stream0, first))
+        z = runtime.to_host(sampling.philox(2, n, seed, stream0, first, normal=True))
+        for r in range(3):
+            assert same_bits(u[r], philox.stream(seed, stream0 + r, first, n))
+        for r in range(2):
+            assert np.abs(z[r] - philox.stream(seed, stream0 + r, first, n, normal=True)).max() < 1e-13
+
+
+def _mps_m8():
+    from bosonsupremacy_amd import instances, mps
+    inst = instances.workload("m8_d4_chi64")
+    G, L, kron_out = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
+    return inst, G, L
+
+
+def test_sampling_m8_matches_oracle():
+    from bosonsupremacy_amd import runtime, sampling
+    inst, G, L = _mps_m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 3000
+    rng = np.random.default_rng(3)
+    normals = rng.normal(size=(2 * inst.M, n))
+    uniforms = rng.uniform(size=(inst.M, n))
+    s = sampling.Sampler(G, L, sqrtW, inst.dd)
+    got, probs = s.run(runtime.to_device(normals), runtime.to_device(uniforms), return_probs=True)
+    got, probs = runtime.to_host(got), runtime.to_host(probs)
+    oG = [runtime.to_host(g) for g in G]
+    want, oprobs = O.sampling(oG, np.stack(L[:-1], axis=1), sqrtW, inst.dd, normals, uniforms)
+    oprobs = np.transpose(oprobs, (1, 0, 2))
+    # conditional distributions agree to FP64 rounding while the draws agree
+    first_diff = np.argmax(np.any(got != want, axis=0)) if np.any(got != want) else inst.M
+    agree = np.all(got == want, axis=1)
+    assert agree.mean() > 0.999
+    for i in range(inst.M):
+        ok = np.all(got[:, :i] == want[:, :i], axis=1)
+        assert np.abs(probs[i][ok] - oprobs[i][ok]).max() < 1e-11, i
+    # a disagreeing draw must sit on a cumulative-probability boundary
+    for s_ in np.where(~agree)[0]:
+        i = int(np.argmax(got[s_] != want[s_]))
+        cdf = np.cumsum(oprobs[i][s_])
+        assert np.min(np.abs(cdf - uniforms[i, s_])) < 1e-9
+    del first_diff
+
+
+def test_sampling_m32_first_order_marginals():
+    """Full-size property: 10^5 samples of the M = 32 workload; the Rao-Blackwellised
+    first-order marginals match the exact lossy-state values (truncation + noise)."""
+    from bosonsupremacy_amd import instances, mps, runtime, sampling
+    inst = instances.workload("m32_d6_chi512")
+    G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    s = sampling.Sampler(G, L, sqrtW, inst.dd)
+    normals, uniforms = sampling.random_inputs(inst.M, inst.n_samples, seed=5)
+    samples, probs = s.run(normals, uniforms, return_probs=True)
+    probs = runtime.to_host(probs)
+    rb = (probs * np.arange(inst.dd)[None, None, :]).sum(axis=2).mean(axis=1)
+    mean, _ = O.photon_number_moments(inst.cov)
+    print("max rel err of E[n_i]:", np.max(np.abs(rb - mean) / mean), flush=True)
+    assert np.all(np.abs(rb - mean) <= 0.05 * mean + 0.01)
+    assert np.all(np.abs(probs.sum(axis=2) - 1) < 1e-12)
+    smp = runtime.to_host(samples)
+    assert smp.min() >= 0 and smp.max() < inst.dd
