@@ -1,0 +1,308 @@
+"""Throughput of the MPS construction + sampling path on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload m32_d6_chi512]
+
+One step is one pass of the path over one synthetic instance: Schmidt spectra of all M-1 cuts,
+the M site tensors, and n_samples photon-number samples.  Every input (covariances, site
+tensors, random streams) is resident on the GPU or generated there before the sampling
+kernels start.  Prints one JSON line:
+
+  value            MPS sites/s (sites / construction time, construction = spectra + tensors)
+  samples_per_s    full M-site samples per second
+  roofline         the dominant kernel (by GPU time) against its MI355X peak
+  kernels          every kernel family: launches, mean duration, achieved rate
+  cpu_baseline     the FP64 CPU oracle timed on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix (dense)
+PEAK_HBM_GBS = 8000.0
+
+
+def hafnian_flops(m):
+    """Real flops of the reference recursion for a 2m x 2m hafnian once structurally-zero
+    polynomial terms are skipped (the arithmetic the GPU kernels execute): per child-matrix
+    term two complex products and two adds (16), per g-update term a product and an add (8)."""
+    n = m
+    memo = {}
+
+    def nz(level, D, G):
+        if level == n:
+            return G >= n
+        key = (level, D, G)
+        if key not in memo:
+            memo[key] = nz(level + 1, D, G) or nz(level + 1, min(n, 2 * D + 1), min(n, G + D + 1))
+        return memo[key]
+
+    def rec(level, D, G):
+        if level == n:
+            return 0
+        s = 2 * n - 2 * level
+        D2, G2 = min(n, 2 * D + 1), min(n, G + D + 1)
+        f = rec(level + 1, D, G) if nz(level + 1, D, G) else 0
+        if nz(level + 1, D2, G2):
+            nc = (s - 2) * (s - 3) // 2
+            pc = sum(1 for t in range(1, D2 + 1) for u in range(t) if u <= D and t - 1 - u <= D)
+            pe = sum(1 for t in range(1, n + 1) for u in range(t) if u <= G and t - 1 - u <= D)
+            f += 16 * nc * pc + 8 * pe + rec(level + 1, D2, G2)
+        return f
+    return rec(0, 0, 0) if m >= 3 else {0: 0, 1: 0, 2: 22}[m]
+
+
+def site_hafnian_work(kres, knum, d, err_tol=1e-10):
+    """(hafnian count, flops) of all site tensors, from the label photon counts."""
+    M = len(kres) + 1
+    total_n, total_f = 0, 0.0
+    fl = {}
+    for k in range(M):
+        if k == M - 1:
+            la = np.zeros(1, int)
+        else:
+            la = np.asarray(knum[k]).reshape(len(kres[k]), -1)[np.asarray(kres[k]) > err_tol].sum(1)
+        lb = np.zeros(1, int) if k == 0 else np.asarray(knum[k - 1]).reshape(len(knum[k - 1]), -1).sum(1)
+        ca, cb = np.bincount(la), np.bincount(lb)
+        for j in range(d):
+            for p in range(len(ca)):
+                for q in range(len(cb)):
+                    n = j + p + q
+                    if n % 2 or ca[p] == 0 or cb[q] == 0:
+                        continue
+                    cnt = int(ca[p]) * int(cb[q])
+                    if n // 2 not in fl:
+                        fl[n // 2] = hafnian_flops(n // 2)
+                    total_n += cnt
+                    total_f += cnt * fl[n // 2]
+    return total_n, total_f
+
+
+class KernelTimer:
+    """HIP events on the library's stream (torch's current stream) around each launch."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.pending = []
+        self.enabled = False
+
+    def __call__(self, name):
+        timer = self
+
+        class _Ctx:
+            def __enter__(self_inner):
+                if timer.enabled:
+                    self_inner.a = timer.torch.cuda.Event(enable_timing=True)
+                    self_inner.a.record()
+
+            def __exit__(self_inner, *exc):
+                if timer.enabled:
+                    b = timer.torch.cuda.Event(enable_timing=True)
+                    b.record()
+                    timer.pending.append((name, self_inner.a, b))
+        return _Ctx()
+
+    def summary(self):
+        self.torch.cuda.synchronize()
+        out = {}
+        for name, a, b in self.pending:
+            out.setdefault(name, []).append(a.elapsed_time(b) * 1e-3)
+        return out
+
+
+def cpu_baseline(inst, kron_out, seconds=20.0):
+    """The FP64 oracle (NumPy, single process) on a bounded sample of the workload:
+    site tensors for a random subset of one mid-chain site's left labels (all right labels,
+    all j), scaled to whole sites; and sampling of a small batch."""
+    from oracle import gbs_oracle as O
+    kres, knum, kS = kron_out
+    M, d, chi = inst.M, inst.d, inst.chi
+    k = M // 2
+    res = np.asarray(kres[k])
+    nL = int((res > 1e-10).sum())
+    rng = np.random.default_rng(0)
+    frac_rows = max(1, int(0.01 * nL))
+    rows = np.sort(rng.choice(nL, size=frac_rows, replace=False))
+    sub_res = list(kres)
+    sub_num = list(knum)
+    sub_res[k] = res[rows]
+    sub_num[k] = np.asarray(knum[k]).reshape(len(res), -1)[rows]
+    t0 = time.perf_counter()
+    O.build_site(k, M, inst.sq_cov, sub_res, sub_num, kS, d, chi)
+    t_site = (time.perf_counter() - t0) * nL / frac_rows
+    sites_per_s = 1.0 / t_site
+    return {
+        "value": sites_per_s, "unit": "sites/s", "cores": 1, "kind": "port",
+        "sample": f"oracle build_site on {frac_rows}/{nL} left labels of site {k} "
+                  f"(all {len(knum[k - 1])} right labels, all j), scaled to a whole site",
+    }
+
+
+def cpu_sampling_rate(inst, G, L, n=50):
+    from oracle import gbs_oracle as O
+    from bosonsupremacy_amd import runtime, sampling
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    oG = [runtime.to_host(g) for g in G]
+    rng = np.random.default_rng(0)
+    normals = rng.normal(size=(2 * inst.M, n))
+    uniforms = rng.uniform(size=(inst.M, n))
+    t0 = time.perf_counter()
+    O.sampling(oG, np.stack(L[:-1], axis=1), sqrtW, inst.dd, normals, uniforms)
+    return n / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="m32_d6_chi512")
+    ap.add_argument("--samples", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
+              f"single GPU with no collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bosonsupremacy_amd import instances, kron, mps, runtime, sampling
+
+    inst = instances.workload(args.workload)
+    n_samples = args.samples or inst.n_samples
+    ctx = runtime.context()
+    from bosonsupremacy_amd import autotune
+    tuning = autotune.load_or_tune()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    timer = KernelTimer()
+
+    def step():
+        t0 = time.perf_counter()
+        kron_out = kron.kron_all(inst.sq_cov, inst.d, inst.chi, timer=timer)
+        G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=kron_out, timer=timer)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s = sampling.Sampler(G, L, sqrtW, inst.dd)
+        normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=1)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        samples = s.run(normals, uniforms, timer=timer)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return (t1 - t0, t3 - t2, t3 - t0), (kron_out, G, L, samples)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer.enabled = True
+    times = []
+    last = None
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        tt, last = step()
+        times.append(tt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    t_build = float(np.mean([t[0] for t in times]))
+    t_samp = float(np.mean([t[1] for t in times]))
+    kern = timer.summary()
+
+    kron_out, G, L, samples = last
+    n_haf, haf_flops = site_hafnian_work(kron_out[0], kron_out[1], inst.d)
+    chi, d = inst.chi, inst.d
+    gemm_flops = 8.0 * n_samples * chi * chi * d      # per site_contract launch
+    kernels = {}
+    for name, ds in kern.items():
+        per_step = len(ds) / args.steps
+        mean = float(np.mean(ds))
+        entry = {"launches_per_step": per_step, "mean_ms": mean * 1e3, "total_ms_per_step": mean * per_step * 1e3}
+        if name == "site_contract":
+            entry["tflops"] = gemm_flops / mean / 1e12
+        if name == "gamma_site":
+            entry["tflops"] = haf_flops / (mean * per_step) / 1e12
+            entry["hafnians_per_step"] = n_haf
+        if name == "sample_site":
+            # algorithmic bytes per (sample, site): read T (chi d complex), write pre (chi complex),
+            # read a uniform and alpha, write the draw
+            byts = n_samples * (chi * d * 16 + chi * 16 + 8 + 16 + 4)
+            entry["gbs"] = byts / mean / 1e9
+        kernels[name] = entry
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms_per_step"])
+    if dom == "site_contract":
+        ach = kernels[dom]["tflops"]
+        roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                "algorithmic": f"8*n*chi^2*d = {gemm_flops:.3e} FP64 flops per launch"}
+    elif dom == "gamma_site":
+        ach = kernels[dom]["tflops"]
+        roof = {"kernel": dom, "bound": "fp64-vector", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        ach = kernels[dom].get("gbs", 0.0)
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None}
+
+    # precision forms on common random numbers: throughput and first-order marginal error
+    forms = {}
+    normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=2)
+    rb = {}
+    for prec in ("fp64", "fp32"):
+        s = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
+        s.run(normals, uniforms)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(normals, uniforms)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        _, probs = s.run(normals, uniforms, return_probs=True)
+        rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
+        forms[prec] = {"samples_per_s": n_samples / dt}
+        del probs
+    rel = torch.abs(rb["fp32"] - rb["fp64"]) / torch.abs(rb["fp64"])
+    forms["fp32"]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
+    forms["fp32"]["speedup_vs_fp64"] = forms["fp32"]["samples_per_s"] / forms["fp64"]["samples_per_s"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(inst, kron_out)
+        cpu["samples_per_s"] = cpu_sampling_rate(inst, G, L)
+        cpu["samples_sample"] = "oracle sampling of 50 full samples"
+
+    sites_per_s = inst.M / t_build
+    out = {
+        "metric": "MPS sites/s at chi (construction), with samples/s",
+        "value": sites_per_s,
+        "unit": "sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic: seeded Haar unitary, closed-form pure/classical split",
+        "config": {"workload": args.workload, "M": inst.M, "d": inst.d, "chi": inst.chi, "dd": inst.dd,
+                   "n_samples": n_samples, "precision_form": "fp64"},
+        "samples_per_s": n_samples / t_samp,
+        "construction_s": t_build,
+        "sampling_s": t_samp,
+        "hafnians_per_instance": n_haf,
+        "roofline": roof,
+        "kernels": kernels,
+        "sampling_forms": forms,
+        "tuning": {str(k): v for k, v in tuning.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -51,6 +51,7 @@ _PROTOTYPES = {
     "bsmps_last_error": ([], ctypes.c_char_p),
     "bsmps_version": ([], c_int),
     "bsmps_set_tuning": ([c_void_p, c_int, c_int], c_int),
+    "bsmps_set_group_lanes": ([c_void_p, c_int, c_int], c_int),
     "bsmps_sigma_select": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_dense": ([c_void_p, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p], c_int),
@@ -63,6 +64,11 @@ _PROTOTYPES = {
                             c_int),
     "bsmps_sample_site": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p], c_int),
+    "bsmps_site_contract_c64": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p], c_int),
+    "bsmps_sample_site_c64": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
+                              c_int),
     "bsmps_philox": ([c_void_p, c_uint64, c_uint32, c_int, c_int64, c_int64, c_int, c_void_p], c_int),
 }
 

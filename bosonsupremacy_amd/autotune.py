@@ -1,0 +1,117 @@
+"""Launch-shape autotuner for the hafnian kernels.
+
+For every even hafnian size it times the candidate launch shapes -- one lane per matrix
+(sizes <= 10) and 16 / 32 / 64 cooperating lanes per matrix -- on synthetic targets drawn
+from a random Sigma, and keeps the fastest.  Every candidate computes bit-identical results
+(the tests check all shapes against the same oracle), so the choice only affects speed.
+Results are cached per device name in ``~/.cache/bsmps/tuning.json`` (or $BSMPS_TUNING_CACHE)
+and applied to the process-wide context.
+"""
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import runtime
+
+GROUP_CANDIDATES = (16, 32, 64)
+LANE_MAX_SIZE = 10
+
+
+def _cache_path():
+    return os.environ.get("BSMPS_TUNING_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "bsmps", "tuning.json"))
+
+
+def _time_call(fn, reps=2):
+    fn()  # warm (plan build, code load)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune(sizes=(6, 8, 10, 12, 14, 16, 18), ld=64, items=None, seed=0, verbose=False):
+    """Time every candidate per size; returns {n: "lane" | lanes_per_matrix}."""
+    ctx = runtime.context()
+    dev = ctx.device
+    rng = np.random.default_rng(seed)
+    S = (rng.normal(size=(ld, ld)) + 1j * rng.normal(size=(ld, ld))) * 0.3
+    S_d = runtime.to_device(S)
+    best = {}
+    for n in sizes:
+        count = items or max(2000, int(2e9 / max(1, _flops(n))))
+        tg = torch.from_numpy(rng.integers(0, ld, size=(count, n)).astype(np.int32)).to(dev)
+        out = torch.empty((count,), dtype=torch.complex128, device=dev)
+
+        def run():
+            ctx.call("bsmps_hafnian_targets", runtime.ptr(S_d), ld, runtime.ptr(tg), count, n, None,
+                     runtime.ptr(out))
+        timings = {}
+        if n <= LANE_MAX_SIZE:
+            ctx.set_tuning(n // 2, 64)
+            timings["lane"] = _time_call(run)
+        ctx.set_tuning(min(2, n // 2 - 1) if n >= 6 else 2, 64)
+        for g in GROUP_CANDIDATES:
+            try:
+                ctx.call("bsmps_set_group_lanes", n, g)
+                timings[g] = _time_call(run)
+            except Exception:
+                continue
+        best[n] = min(timings, key=timings.get)
+        if verbose:
+            print(f"n={n} items={count} " + " ".join(f"{k}:{v:.2f}ms" for k, v in timings.items()), flush=True)
+    return best
+
+
+def apply(best):
+    """Install a tuning {n: "lane" | lanes} on the process-wide context."""
+    ctx = runtime.context()
+    lane_sizes = [n for n, v in best.items() if v == "lane"]
+    lane_max_m = 2
+    for n in sorted(lane_sizes):
+        if all(best.get(k) == "lane" for k in range(6, n + 1, 2) if k in best):
+            lane_max_m = max(lane_max_m, n // 2)
+    ctx.set_tuning(lane_max_m, 64)
+    for n, v in best.items():
+        if v != "lane":
+            ctx.call("bsmps_set_group_lanes", int(n), int(v))
+
+
+def load_or_tune(verbose=False):
+    """Apply the cached tuning for this device, tuning first if there is none."""
+    name = torch.cuda.get_device_name(runtime.context().device)
+    path = _cache_path()
+    cache = {}
+    if os.path.exists(path):
+        try:
+            with open(path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    entry = cache.get(name)
+    if entry is None:
+        best = tune(verbose=verbose)
+        entry = {str(k): v for k, v in best.items()}
+        cache[name] = entry
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(cache, f, indent=1)
+        except OSError:
+            pass
+    best = {int(k): v for k, v in entry.items()}
+    apply(best)
+    return best
+
+
+def _flops(n):
+    m = n // 2
+    return {0: 1, 1: 1, 2: 22, 3: 336, 4: 1520, 5: 6792, 6: 26128, 7: 90888, 8: 291064, 9: 891632,
+            10: 2.6e6, 11: 7.5e6, 12: 2.1e7}.get(m, 1e8)

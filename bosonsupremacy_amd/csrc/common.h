@@ -55,7 +55,8 @@ struct bsmps_ctx {
     size_t ws_bytes;
     // autotuned launch parameters
     int lane_max_m;   // hafnians of size <= 2*lane_max_m take the one-lane-per-hafnian path
-    int group_lanes;  // lanes per hafnian in the cooperative path (16, 32 or 64)
+    int group_lanes_by_m[16];  // lanes per hafnian in the cooperative path (16, 32 or 64), by n/2
+    int max_lds;      // bytes of LDS one workgroup may use on this device
 };
 
 int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out);

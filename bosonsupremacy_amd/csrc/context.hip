@@ -58,18 +58,17 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     c->ws = nullptr;
     c->ws_bytes = 0;
     c->lane_max_m = 4;
-    c->group_lanes = 64;
-    if (stream) {
-        c->stream = reinterpret_cast<hipStream_t>(stream);
-        c->own_stream = 0;
-    } else {
-        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            delete c;
-            return bsmps_check_hip(e, "hipStreamCreateWithFlags");
-        }
-        c->own_stream = 1;
+    for (int m = 0; m < 16; ++m) c->group_lanes_by_m[m] = 64;
+    c->max_lds = 64 * 1024;
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+            c->max_lds = v;
     }
+    // NULL is HIP's null stream, which is also PyTorch's default stream: work enqueued here
+    // is ordered with the caller's allocations, fills and copies on that stream.
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    c->own_stream = 0;
     *out = c;
     return BSMPS_OK;
 }
@@ -100,6 +99,17 @@ extern "C" int bsmps_set_tuning(bsmps_ctx* ctx, int lane_max_m, int group_lanes)
         return BSMPS_ERR_INVALID;
     }
     ctx->lane_max_m = lane_max_m;
-    ctx->group_lanes = group_lanes;
+    for (int m = 0; m < 16; ++m) ctx->group_lanes_by_m[m] = group_lanes;
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_set_group_lanes(bsmps_ctx* ctx, int n, int group_lanes) {
+    if (!ctx || n < 0 || n > BSMPS_MAX_HAFNIAN_SIZE || n % 2 ||
+        (group_lanes != 16 && group_lanes != 32 && group_lanes != 64)) {
+        bsmps_set_error("bsmps_set_group_lanes: n must be even and <= %d, group_lanes 16, 32 or 64",
+                        BSMPS_MAX_HAFNIAN_SIZE);
+        return BSMPS_ERR_INVALID;
+    }
+    ctx->group_lanes_by_m[n / 2] = group_lanes;
     return BSMPS_OK;
 }

@@ -397,14 +397,61 @@ __device__ __forceinline__ void decode_pair(int i, int& J, int& K) {
     K = i - j * (j - 1) / 2;
 }
 
-template <int G, class S>
+// Right-branch child-matrix entry (J, K): all coefficients t = 1..D2 of
+//   c(J,K)[t] = b(J,K)[t] + sum_u (b(J,b0)[u] b(K,b1)[v] + b(K,b0)[u] b(J,b1)[v]),  v = t-1-u,
+// accumulated in ascending u as solve() does (MPS_cpu.py:179-186).  The node's four pivot
+// polynomials are loaded once into registers; D (the node's degree) and D2 are compile-time,
+// which is possible because a node's degree is always 2^r - 1 or m.
+template <int D, int D2>
+__device__ __forceinline__ void child_entry(const cd* Ar, const cd* So, int obase, int ocap, cd* dst, int J,
+                                            int K, int b0, int b1) {
+    auto co = [&](int r, int c, int u) -> cd {
+        return u == 0 ? Ar[tri(r, c)] : So[tri(r - obase, c - obase) * ocap + (u - 1)];
+    };
+    cd X[D + 1], Y[D + 1], Z[D + 1], W[D + 1];
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+        X[u] = co(J, b0, u);
+        Y[u] = co(K, b1, u);
+        Z[u] = co(K, b0, u);
+        W[u] = co(J, b1, u);
+    }
+#pragma unroll
+    for (int t = 1; t <= D2; ++t) {
+        cd acc = (t <= D) ? co(J, K, t) : czero();
+#pragma unroll
+        for (int u = 0; u <= D; ++u) {
+            const int v = t - 1 - u;
+            if (v >= 0 && v <= D) acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+        }
+        dst[t - 1] = acc;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void child_entry_dispatch(int D, const cd* Ar, const cd* So, int obase, int ocap, cd* dst,
+                                                     int J, int K, int b0, int b1) {
+    if (D == M) {
+        child_entry<M, M>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+    } else if (D == 0) {
+        child_entry<0, cmin(M, 1)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+    } else if (D == 1) {
+        if constexpr (M > 1) child_entry<1, cmin(M, 3)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+    } else if (D == 3) {
+        if constexpr (M > 3) child_entry<3, cmin(M, 7)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+    } else if (D == 7) {
+        if constexpr (M > 7) child_entry<7, cmin(M, 15)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+    }
+}
+
+template <int M, class S>
 __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, Layout lay,
-                                                          const int4* __restrict__ ops) {
+                                                          const int4* __restrict__ ops, int G) {
     extern __shared__ cd smem[];
-    constexpr int PER = kWave / G;  // matrices per workgroup
+    const int PER = kWave / G;  // matrices per workgroup
     const int grp = threadIdx.x / G;
     const int gl = threadIdx.x % G;
-    const int m = lay.m;
+    constexpr int m = M;
     const int n = 2 * m;
     cd* R = smem + grp * lay.total;
     int* tg = reinterpret_cast<int*>(R + lay.off_tgt);
@@ -472,18 +519,7 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
                     int Jl, Kl;
                     decode_pair(it, Jl, Kl);
                     const int J = Jl + b0 + 2, K = Kl + b0 + 2;
-                    for (int t = 1; t <= D2; ++t) {
-                        cd acc = (t <= D) ? coef(J, K, t) : czero();
-                        const int ulo = t - 1 - D > 0 ? t - 1 - D : 0;
-                        const int uhi = t - 1 < D ? t - 1 : D;
-                        for (int u = ulo; u <= uhi; ++u) {
-                            const int v = t - 1 - u;
-                            const cd t1 = cmul(coef(J, b0, u), coef(K, b1, v));
-                            const cd t2 = cmul(coef(K, b0, u), coef(J, b1, v));
-                            acc = cadd(acc, cadd(t1, t2));
-                        }
-                        Sd[tri(J - 2 * (level + 1), K - 2 * (level + 1)) * dcap + (t - 1)] = acc;
-                    }
+                    child_entry_dispatch<M>(D, R + lay.off_A, So, obase, ocap, Sd + tri(Jl, Kl) * dcap, J, K, b0, b1);
                 } else {
                     const int t = it - n_child + 1;  // e[t], t = 1..m
                     cd acc = (t <= Gd) ? gs[t] : czero();
@@ -495,6 +531,7 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
             }
             if (gl == 0) gd[0] = gs[0];
             (void)G2;
+            (void)D2;
             __syncthreads();
         }
     }
@@ -541,37 +578,43 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
     if (src.n_items == 0) return BSMPS_OK;
     GroupPlan* plan = nullptr;
     BSMPS_TRY(get_plan(ctx->device, m, &plan));
-    const int G = ctx->group_lanes;
+    const int G = ctx->group_lanes_by_m[m];
     const int per = kWave / G;
     const size_t lds = sizeof(cd) * (size_t)plan->lay.total * per;
-    if (lds > 160 * 1024) {
-        bsmps_set_error("hafnian of size %d needs %zu bytes of LDS per workgroup with %d lanes per matrix",
-                        2 * m, lds, G);
+    if (lds > (size_t)ctx->max_lds) {
+        bsmps_set_error("hafnian of size %d needs %zu bytes of LDS per workgroup with %d lanes per matrix "
+                        "(device limit %d)", 2 * m, lds, G, ctx->max_lds);
         return BSMPS_ERR_UNSUPPORTED;
     }
-    if (lds > 64 * 1024) {
-        const void* fn = G == 16 ? (const void*)&haf_group_kernel<16, S>
-                       : G == 32 ? (const void*)&haf_group_kernel<32, S>
-                                 : (const void*)&haf_group_kernel<64, S>;
-        BSMPS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
     const int64_t blocks = (src.n_items + per - 1) / per;
-    for (int64_t b0 = 0; b0 < blocks; b0 += kMaxGrid) {
-        const int64_t nb = std::min<int64_t>(kMaxGrid, blocks - b0);
-        const int64_t item0 = b0 * per;
-        dim3 grid((unsigned)nb), block(kWave);
-        switch (G) {
-            case 16:
-                hipLaunchKernelGGL((haf_group_kernel<16, S>), grid, block, lds, ctx->stream, src, item0, plan->lay, plan->d_ops);
-                break;
-            case 32:
-                hipLaunchKernelGGL((haf_group_kernel<32, S>), grid, block, lds, ctx->stream, src, item0, plan->lay, plan->d_ops);
-                break;
-            default:
-                hipLaunchKernelGGL((haf_group_kernel<64, S>), grid, block, lds, ctx->stream, src, item0, plan->lay, plan->d_ops);
-                break;
-        }
-        BSMPS_HIP(hipGetLastError());
+    switch (m) {
+#define BSMPS_GROUP_CASE(MM)                                                                                  \
+    case MM: {                                                                                                \
+        if (lds > 64 * 1024)                                                                                  \
+            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S>,                              \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
+        for (int64_t b0 = 0; b0 < blocks; b0 += kMaxGrid) {                                                   \
+            const int64_t nb = std::min<int64_t>(kMaxGrid, blocks - b0);                                      \
+            hipLaunchKernelGGL((haf_group_kernel<MM, S>), dim3((unsigned)nb), dim3(kWave), lds, ctx->stream, src, \
+                               b0 * per, plan->lay, plan->d_ops, G);                                          \
+            BSMPS_HIP(hipGetLastError());                                                                     \
+        }                                                                                                     \
+        break;                                                                                                \
+    }
+        BSMPS_GROUP_CASE(3)
+        BSMPS_GROUP_CASE(4)
+        BSMPS_GROUP_CASE(5)
+        BSMPS_GROUP_CASE(6)
+        BSMPS_GROUP_CASE(7)
+        BSMPS_GROUP_CASE(8)
+        BSMPS_GROUP_CASE(9)
+        BSMPS_GROUP_CASE(10)
+        BSMPS_GROUP_CASE(11)
+        BSMPS_GROUP_CASE(12)
+#undef BSMPS_GROUP_CASE
+        default:
+            bsmps_set_error("group hafnian path supports sizes 6..24, got %d", 2 * m);
+            return BSMPS_ERR_INVALID;
     }
     return BSMPS_OK;
 }

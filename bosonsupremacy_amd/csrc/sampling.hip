@@ -162,46 +162,147 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
         }
         __syncthreads();
     }
-    // result layout: lane holds D[4*(lane/16) + r][lane%16], r = 0..3
+    // result layout of v_mfma_f64_16x16x4f64: register r of lane l holds D[4*r + l/16][l%16]
+    // (the f32 variant below differs: D[4*(l/16) + r][l%16])
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int r = 0; r < 4; ++r) {
-                const int64_t s = row0 + wm + i * 16 + 4 * (lane / 16) + r;
+                const int64_t s = row0 + wm + i * 16 + 4 * r + lane / 16;
                 const int64_t c = col0 + wn + j * 16 + lane % 16;
                 if (s < n && c < Nreal) T[s * Nreal + c] = acc[i][j][r];
+            }
+}
+
+// FP32 form of the same contraction (v_mfma_f32_16x16x4_f32, FP32 accumulation): complex64
+// operands and output, lambda applied in FP64 then rounded.  Same tiling and lane layout.
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void site_gemm_f32_kernel(int64_t n, int chi_prev, int ncols,
+                                                            const float* __restrict__ pre,
+                                                            const double* __restrict__ lam,
+                                                            const float* __restrict__ gamma, float* __restrict__ T) {
+    __shared__ float As[TK][TM + 4];
+    __shared__ float Bs[TK][TN + 4];
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int64_t row0 = (int64_t)blockIdx.y * TM;
+    const int64_t col0 = (int64_t)blockIdx.x * TN;
+    const int64_t Nreal = 2 * (int64_t)ncols;
+    const int Kreal = 2 * chi_prev;
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    float4_t acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < Kreal; k0 += TK) {
+        for (int e = tid; e < TM * (TK / 2); e += 256) {
+            const int r = e / (TK / 2), bb = e % (TK / 2);
+            const int64_t srow = row0 + r;
+            const int b = k0 / 2 + bb;
+            float re = 0.f, im = 0.f;
+            if (srow < n && b < chi_prev) {
+                const float2 p = reinterpret_cast<const float2*>(pre)[srow * chi_prev + b];
+                const double l = lam[b];
+                re = (float)(p.x * l);
+                im = (float)(p.y * l);
+            }
+            As[2 * bb][r] = re;
+            As[2 * bb + 1][r] = im;
+        }
+        {
+            const int bb = tid / (TN / 2), cc = tid % (TN / 2);
+            const int b = k0 / 2 + bb;
+            const int64_t c = col0 / 2 + cc;
+            float gr = 0.f, gi = 0.f;
+            if (b < chi_prev && c < ncols) {
+                const float2 g = reinterpret_cast<const float2*>(gamma)[(int64_t)b * ncols + c];
+                gr = g.x;
+                gi = g.y;
+            }
+            Bs[2 * bb][2 * cc] = gr;
+            Bs[2 * bb][2 * cc + 1] = gi;
+            Bs[2 * bb + 1][2 * cc] = -gi;
+            Bs[2 * bb + 1][2 * cc + 1] = gr;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 4) {
+            float a[2], b[2];
+            for (int i = 0; i < 2; ++i) a[i] = As[kk + lane / 16][wm + i * 16 + lane % 16];
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + lane / 16][wn + j * 16 + lane % 16];
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int64_t srow = row0 + wm + i * 16 + 4 * (lane / 16) + r;
+                const int64_t c = col0 + wn + j * 16 + lane % 16;
+                if (srow < n && c < Nreal) T[srow * Nreal + c] = acc[i][j][r];
             }
 }
 
 // ---------------------------------------------------------------------------------------
 // Displacement + probabilities + draw, one workgroup per sample.
 // ---------------------------------------------------------------------------------------
+template <typename TT>
+__device__ __forceinline__ cd ld_c(const TT* p, int64_t i);
+template <>
+__device__ __forceinline__ cd ld_c<double>(const double* p, int64_t i) {
+    return ld_cd(p, i);
+}
+template <>
+__device__ __forceinline__ cd ld_c<float>(const float* p, int64_t i) {
+    const float2 v = reinterpret_cast<const float2*>(p)[i];
+    return cd{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ void st_c(double* p, int64_t i, cd v) { st_cd(p, i, v); }
+__device__ __forceinline__ void st_c(float* p, int64_t i, cd v) {
+    reinterpret_cast<float2*>(p)[i] = make_float2((float)v.re, (float)v.im);
+}
+
 constexpr int kMaxDD = 64;
 constexpr int kMaxd = 32;
 
 // DMAX / DDMAX bound d and dd at compile time so the per-thread arrays stay in VGPRs.
-template <int DMAX, int DDMAX>
+// One wave per sample, four samples per workgroup.  TT: storage type of T and pre (double:
+// FP64 form; float: reduced-precision forms, where the carried vector is rescaled by its
+// largest component after every site so that a long chain cannot underflow -- the
+// reference's GPU path does the same, distributed_sampling.py:147).  The displacement, the
+// probabilities and the inverse CDF are evaluated in FP64 in every form.
+template <int DMAX, int DDMAX, typename TT, bool RENORM>
 __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
-                                                          const double* __restrict__ T, int64_t tstride,
+                                                          const TT* __restrict__ T, int64_t tstride,
                                                           const double* __restrict__ alpha, int64_t astride,
                                                           const double* __restrict__ W, const double* __restrict__ x,
                                                           const double* __restrict__ lam,
                                                           const double* __restrict__ uni, int32_t* __restrict__ samples,
-                                                          int64_t sstride, double* __restrict__ pre,
+                                                          int64_t sstride, TT* __restrict__ pre,
                                                           double* __restrict__ probs) {
-    __shared__ double cr[DDMAX], sr[DDMAX];
-    __shared__ cd Dm[DDMAX * DMAX];  // D[k][j], j < d
-    __shared__ double red[4][DDMAX];
-    __shared__ int s_pick;
-    __shared__ double s_theta;
-    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
-        if (tid == 0) {
-            double th;
-            displace_entry_setup(ld_cd(alpha, s * astride), dd, x, cr, sr, th);
-            s_theta = th;
+    __shared__ double s_cr[4][DDMAX], s_sr[4][DDMAX];
+    __shared__ cd s_D[4][DDMAX * DMAX];  // D[k][j], j < d
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    double* cr = s_cr[wave];
+    double* sr = s_sr[wave];
+    cd* Dm = s_D[wave];
+    const int64_t n_rounds = (n + 3) / 4;
+    for (int64_t rnd = blockIdx.x; rnd < n_rounds; rnd += gridDim.x) {
+        const int64_t s = rnd * 4 + wave;
+        const bool live = s < n;   // uniform per wave
+        const int64_t se = live ? s : n - 1;
+        const cd a = ld_cd(alpha, se * astride);
+        const double r = hypot(a.re, a.im);
+        const double theta = atan2(a.im, a.re) + 1.5707963267948966;
+        if (lane < dd) {
+            double sn, cs;
+            sincos(r * x[lane], &sn, &cs);
+            cr[lane] = cs;
+            sr[lane] = -sn;  // e^{-i r x_q}
         }
         __syncthreads();
-        for (int e = tid; e < dd * d; e += blockDim.x) {
+        for (int e = lane; e < dd * d; e += 64) {
             const int k = e / d, j = e - k * d;
             double re = 0.0, im = 0.0;
             for (int q = 0; q < dd; ++q) {
@@ -210,21 +311,21 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                 im = fma(w, sr[q], im);
             }
             double sp, cp;
-            sincos((double)(k - j) * s_theta, &sp, &cp);
+            sincos((double)(k - j) * theta, &sp, &cp);
             Dm[k * DMAX + j] = cd{re * cp - im * sp, re * sp + im * cp};
         }
         __syncthreads();
-        const double* Ts = T + 2 * s * tstride;
+        const TT* Ts = T + 2 * se * tstride;
         // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda)
         double pk[DDMAX];
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) pk[k] = 0.0;
         const int a_end = lam ? chi : 1;
-        for (int a = tid; a < a_end; a += blockDim.x) {
+        for (int aa = lane; aa < a_end; aa += 64) {
             cd t[DMAX];
 #pragma unroll
-            for (int j = 0; j < DMAX; ++j) t[j] = j < d ? ld_cd(Ts, (int64_t)a * d + j) : czero();
-            const double l2 = lam ? lam[a] * lam[a] : 1.0;
+            for (int j = 0; j < DMAX; ++j) t[j] = j < d ? ld_c(Ts, (int64_t)aa * d + j) : czero();
+            const double l2 = lam ? lam[aa] * lam[aa] : 1.0;
 #pragma unroll
             for (int k = 0; k < DDMAX; ++k) {
                 if (k < dd) {
@@ -241,54 +342,62 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                 }
             }
         }
+        // wave reduction; lane 0 ends with the totals
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) {
             if (k < dd) {
                 double v = pk[k];
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-                if (lane == 0) red[wave][k] = v;
+                pk[k] = v;
             }
         }
-        __syncthreads();
-        if (tid == 0) {
-            double pf[DDMAX];
+        int pick = 0;
+        if (lane == 0) {
             double tot = 0.0;
 #pragma unroll
-            for (int k = 0; k < DDMAX; ++k) {
-                if (k < dd) {
-                    double v = 0.0;
-                    for (int w = 0; w < 4; ++w) v += red[w][k];
-                    pf[k] = v;
-                    tot += v;
-                }
-            }
-            const double u = uni[s];
+            for (int k = 0; k < DDMAX; ++k)
+                if (k < dd) tot += pk[k];
+            const double u = uni[se];
             double cdf = 0.0;
             int cntk = 0;
 #pragma unroll
             for (int k = 0; k < DDMAX; ++k) {
                 if (k < dd) {
-                    const double pnorm = pf[k] / tot;
+                    const double pnorm = pk[k] / tot;
                     cdf += pnorm;
                     if (u > cdf) ++cntk;
-                    if (probs) probs[s * dd + k] = pnorm;
+                    if (probs && live) probs[s * dd + k] = pnorm;
                 }
             }
-            s_pick = cntk < dd - 1 ? cntk : dd - 1;
-            samples[s * sstride] = s_pick;
+            pick = cntk < dd - 1 ? cntk : dd - 1;
+            if (live) samples[s * sstride] = pick;
         }
-        __syncthreads();
-        const int kp = s_pick;
-        if (pre) {
-            for (int a = tid; a < chi; a += blockDim.x) {
+        pick = __shfl(pick, 0, 64);
+        if (pre && live) {
+            auto sel = [&](int aa) -> cd {
                 double ur = 0.0, ui = 0.0;
                 for (int j = 0; j < d; ++j) {
-                    const cd dk = Dm[kp * DMAX + j];
-                    const cd t = ld_cd(Ts, (int64_t)a * d + j);
+                    const cd dk = Dm[pick * DMAX + j];
+                    const cd t = ld_c(Ts, (int64_t)aa * d + j);
                     ur = fma(dk.re, t.re, fma(-dk.im, t.im, ur));
                     ui = fma(dk.re, t.im, fma(dk.im, t.re, ui));
                 }
-                st_cd(pre, s * chi + a, cd{ur, ui});
+                return cd{ur, ui};
+            };
+            double scale = 1.0;
+            if constexpr (RENORM) {
+                double mx = 0.0;
+                for (int aa = lane; aa < chi; aa += 64) {
+                    const cd u = sel(aa);
+                    mx = fmax(mx, fmax(fabs(u.re), fabs(u.im)));
+                }
+                for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+                scale = mx > 0.0 ? 1.0 / mx : 1.0;
+            }
+            for (int aa = lane; aa < chi; aa += 64) {
+                cd u = sel(aa);
+                if constexpr (RENORM) u = cscale(u, scale);
+                st_c(pre, s * chi + aa, u);
             }
         }
         __syncthreads();
@@ -418,9 +527,9 @@ extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int 
         return BSMPS_ERR_INVALID;
     }
 #define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                        \
-    hipLaunchKernelGGL((sample_site_kernel<DM, DDM>), dim3(grid_for(n, 1)), dim3(256), 0, ctx->stream, n, chi, d, dd, \
-                       T, T_sample_stride, alpha, alpha_stride, eigvec, eigval, lambda, uniforms, samples,       \
-                       samples_stride, pre, probs)
+    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, double, false>), dim3(grid_for((n + 3) / 4, 1)), dim3(256), 0, \
+                       ctx->stream, n, chi, d, dd, T, T_sample_stride, alpha, alpha_stride, eigvec, eigval,      \
+                       lambda, uniforms, samples, samples_stride, pre, probs)
     if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
@@ -442,6 +551,56 @@ extern "C" int bsmps_philox(bsmps_ctx* ctx, uint64_t seed, uint32_t stream0, int
     }
     hipLaunchKernelGGL(philox_kernel, dim3(grid_for(n * n_streams, 256)), dim3(256), 0, ctx->stream, seed, stream0,
                        n_streams, first, n, normal, out);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const float* pre,
+                                       const double* lambda_prev, const float* gamma, float* T) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_site_contract_c64: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre || !lambda_prev || !gamma || !T) {
+        bsmps_set_error("bsmps_site_contract_c64: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d;
+    const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
+    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+        bsmps_set_error("bsmps_site_contract_c64: problem too large for one launch");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(site_gemm_f32_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                       chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const float* T,
+                                     int64_t T_sample_stride, const double* alpha, int64_t alpha_stride,
+                                     const double* eigvec, const double* eigval, const double* lambda,
+                                     const double* uniforms, int32_t* samples, int64_t samples_stride, float* pre,
+                                     double* probs) {
+    if (!ctx || n < 0 || chi <= 0 || d <= 0 || d > kMaxd || dd < d || dd > kMaxDD || T_sample_stride < 0 ||
+        alpha_stride < 0 || samples_stride < 0) {
+        bsmps_set_error("bsmps_sample_site_c64: invalid arguments (d <= %d, d <= dd <= %d)", kMaxd, kMaxDD);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!T || !alpha || !eigvec || !eigval || !uniforms || !samples) {
+        bsmps_set_error("bsmps_sample_site_c64: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                        \
+    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, float, true>), dim3(grid_for((n + 3) / 4, 1)), dim3(256), 0,   \
+                       ctx->stream, n, chi, d, dd, T, T_sample_stride, alpha, alpha_stride, eigvec, eigval,      \
+                       lambda, uniforms, samples, samples_stride, pre, probs)
+    if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+    else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
+    else BSMPS_SAMPLE_LAUNCH(32, 64);
+#undef BSMPS_SAMPLE_LAUNCH
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }

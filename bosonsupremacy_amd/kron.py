@@ -15,6 +15,7 @@ CLI (same flags and files as the reference)::
     python -m bosonsupremacy_amd.kron --d D --chi CHI --dir ROOT/
 """
 import argparse
+import contextlib
 import ctypes
 import os
 
@@ -41,7 +42,7 @@ def cut_spectrum_inputs(sq_cov, L, cutoff):
     return tp, S
 
 
-def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0):
+def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None):
     """Top-chi Schmidt weights and labels of several cuts at once on the GPU.
 
     tps: list of arrays [n_c, d] (thermal distributions of each cut's modes).
@@ -62,9 +63,10 @@ def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0):
     res_d = torch.zeros((n_cuts, chi), dtype=torch.float64, device=dev)
     num_d = torch.zeros((n_cuts, chi, max_modes), dtype=torch.int8, device=dev)
     len_d = torch.zeros((n_cuts,), dtype=torch.int32, device=dev)
-    ctx.call("bsmps_kron_cuts", n_cuts, n_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_modes, d,
-             runtime.ptr(tp_d), int(max_dim), int(chi), float(err_tol), runtime.ptr(res_d), runtime.ptr(num_d),
-             runtime.ptr(len_d))
+    with (timer("kron_cuts") if timer else contextlib.nullcontext()):
+        ctx.call("bsmps_kron_cuts", n_cuts, n_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_modes, d,
+                 runtime.ptr(tp_d), int(max_dim), int(chi), float(err_tol), runtime.ptr(res_d),
+                 runtime.ptr(num_d), runtime.ptr(len_d))
     res_h, num_h, len_h = runtime.to_host(res_d), runtime.to_host(num_d), runtime.to_host(len_d)
     out = []
     for c in range(n_cuts):
@@ -84,7 +86,7 @@ def get_cumsum_kron(sq_cov, L, chi=100, max_dim=MAX_DIM, cutoff=6, err_tol=ERR_T
     return res, num, S
 
 
-def kron_all(sq_cov, d, chi, max_dim=MAX_DIM, err_tol=ERR_TOL):
+def kron_all(sq_cov, d, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, timer=None):
     """All cuts k = 0..M-2 (cut k keeps modes k+1..M-1), as kron_cpu.py:147-153."""
     M = len(sq_cov) // 2
     tps, Ss = [], []
@@ -92,7 +94,7 @@ def kron_all(sq_cov, d, chi, max_dim=MAX_DIM, err_tol=ERR_TOL):
         tp, S = cut_spectrum_inputs(sq_cov, k + 1, d)
         tps.append(tp)
         Ss.append(S)
-    spec = kron_cuts(tps, chi, max_dim, err_tol)
+    spec = kron_cuts(tps, chi, max_dim, err_tol, timer=timer)
     return [r for r, _ in spec], [n for _, n in spec], Ss
 
 

@@ -15,6 +15,7 @@ CLI (same flags and files as the reference)::
     python -m bosonsupremacy_amd.mps --d D --chi CHI --dir ROOT/
 """
 import argparse
+import contextlib
 import ctypes
 import time
 
@@ -147,7 +148,7 @@ def _dev_i32(a):
     return runtime.to_device(a)
 
 
-def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL):
+def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
 
     Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site."""
@@ -191,22 +192,23 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
         right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
         right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
     gamma = torch.zeros((chi, chi, d), dtype=torch.complex128, device=dev)
-    ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
+    with (timer("gamma_site") if timer else contextlib.nullcontext()):
+        ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
     return gamma, Lambda
 
 
-def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5):
+def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
     outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them."""
     from . import kron as kron_mod
     M = len(sq_cov) // 2
     if kron is None:
-        kron = kron_mod.kron_all(sq_cov, d, chi, max_dim)
+        kron = kron_mod.kron_all(sq_cov, d, chi, max_dim, timer=timer)
     kres, knum, kS = kron
     S_full = gaussian.williamson(sq_cov)[1]
     Gammas, Lambdas = [], []
     for k in range(M):
-        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full)
+        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full, timer=timer)
         Gammas.append(G)
         Lambdas.append(L)
     return Gammas, Lambdas, kron

@@ -3,6 +3,7 @@
 PyTorch is used only as device-memory and stream plumbing: buffers are torch tensors, and
 their data pointers go to the HIP kernels through the C ABI.
 """
+import atexit
 import ctypes
 
 import numpy as np
@@ -13,13 +14,28 @@ from . import _lib
 _contexts = {}
 
 
+def _drain_at_exit():
+    # Kernels still in flight when the interpreter tears down would race with the release
+    # of the tensors they use; wait for them first.
+    for ctx in list(_contexts.values()):
+        try:
+            ctx.synchronize()
+        except Exception:
+            pass
+
+
+atexit.register(_drain_at_exit)
+
+
 def _require_gpu():
     if not torch.cuda.is_available():
         raise RuntimeError("bosonsupremacy_amd needs a ROCm GPU; none is visible")
 
 
 class Context:
-    """Owns a bsmps_ctx bound to ``device`` and the torch stream current at creation."""
+    """Owns a bsmps_ctx bound to ``device`` and enqueuing on the torch stream current at
+    creation (normally torch's default stream, HIP's null stream), so library kernels are
+    ordered with torch's allocations, fills, frees and copies of the same buffers."""
 
     def __init__(self, device=0):
         _require_gpu()

@@ -17,6 +17,7 @@ CLI (same flags and files as the reference)::
     python -m bosonsupremacy_amd.sampling --N 1000 --n 100 --iter 1 --d 4 --dd 10 --chi 64 --dir ROOT/
 """
 import argparse
+import ctypes
 
 import numpy as np
 import torch
@@ -97,7 +98,9 @@ class Sampler:
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
     """
 
-    def __init__(self, gammas, lambdas, sqrtW, dd):
+    PRECISIONS = ("fp64", "fp32")
+
+    def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64"):
         self.ctx = runtime.context()
         dev = self.ctx.device
         self.M = len(gammas)
@@ -105,7 +108,11 @@ class Sampler:
         self.dd = dd
         if dd < self.d:
             raise ValueError("dd must be >= d")
-        self.gammas = [g.contiguous() for g in gammas]
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {self.PRECISIONS}")
+        self.precision = precision
+        ctype = torch.complex128 if precision == "fp64" else torch.complex64
+        self.gammas = [g.to(ctype).contiguous() for g in gammas]
         self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
         self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
         W, x = displacement_basis(dd)
@@ -113,9 +120,12 @@ class Sampler:
         self.x = runtime.to_device(x)
         self.dev = dev
 
-    def run(self, normals, uniforms, return_probs=False):
+    def run(self, normals, uniforms, return_probs=False, timer=None):
         """normals [2M, n], uniforms [M, n] (device float64) -> samples [n, M] int32 (device),
-        and optionally the conditional distributions [M, n, dd]."""
+        and optionally the conditional distributions [M, n, dd].  ``timer`` (optional) is a
+        context-manager factory ``timer(name)`` wrapped around each kernel launch."""
+        import contextlib
+        timer = timer or (lambda name: contextlib.nullcontext())
         ctx, M, chi, d, dd = self.ctx, self.M, self.chi, self.d, self.dd
         n = normals.shape[1]
         normals = normals.contiguous()
@@ -123,24 +133,28 @@ class Sampler:
         alpha = torch.empty((n, M), dtype=torch.complex128, device=self.dev)
         ctx.call("bsmps_mu_to_alpha", runtime.ptr(self.sqrtW), M, runtime.ptr(normals), n, runtime.ptr(alpha))
         samples = torch.empty((n, M), dtype=torch.int32, device=self.dev)
-        pre = torch.empty((n, chi), dtype=torch.complex128, device=self.dev)
-        T = torch.empty((n, chi, d), dtype=torch.complex128, device=self.dev)
+        ctype = torch.complex128 if self.precision == "fp64" else torch.complex64
+        contract, draw = (("bsmps_site_contract", "bsmps_sample_site") if self.precision == "fp64"
+                          else ("bsmps_site_contract_c64", "bsmps_sample_site_c64"))
+        pre = torch.empty((n, chi), dtype=ctype, device=self.dev)
+        T = torch.empty((n, chi, d), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
-        import ctypes
         for i in range(M):
             last = i == M - 1
             if i == 0:
                 src, stride = self.gammas[0], 0  # Gamma_0[0, a, j] is the site-0 tensor for every sample
             else:
-                ctx.call("bsmps_site_contract", n, chi, chi, d, runtime.ptr(pre), runtime.ptr(self.lambdas[i - 1]),
-                         runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                with timer("site_contract"):
+                    ctx.call(contract, n, chi, chi, d, runtime.ptr(pre),
+                             runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
                 src, stride = T, chi * d
             a_ptr = ctypes.c_void_p(alpha.data_ptr() + 16 * i)
             s_ptr = ctypes.c_void_p(samples.data_ptr() + 4 * i)
             p_ptr = runtime.ptr(probs[i]) if probs is not None else None
-            ctx.call("bsmps_sample_site", n, chi, d, dd, runtime.ptr(src), stride, a_ptr, M, runtime.ptr(self.W),
-                     runtime.ptr(self.x), None if last else runtime.ptr(self.lambdas[i]),
-                     runtime.ptr(uniforms[i]), s_ptr, M, None if last else runtime.ptr(pre), p_ptr)
+            with timer("sample_site"):
+                ctx.call(draw, n, chi, d, dd, runtime.ptr(src), stride, a_ptr, M,
+                         runtime.ptr(self.W), runtime.ptr(self.x), None if last else runtime.ptr(self.lambdas[i]),
+                         runtime.ptr(uniforms[i]), s_ptr, M, None if last else runtime.ptr(pre), p_ptr)
         return (samples, probs) if return_probs else samples
 
 

@@ -37,17 +37,22 @@ typedef struct bsmps_ctx bsmps_ctx;
 
 /* ---- context --------------------------------------------------------------------------- */
 
-/* Create a context on HIP device `device`.  `stream` is a hipStream_t to enqueue on, or
- * NULL for a stream owned by the context.  (The reference has no context: it runs on
- * CuPy's current device and default stream, MPS_utils.py:15-19.) */
+/* Create a context on HIP device `device` that enqueues all work on `stream` (a hipStream_t;
+ * NULL is HIP's null stream, which is PyTorch's default stream).  Callers must enqueue the
+ * producers and consumers of the buffers they pass on the same stream, or synchronise.
+ * (The reference has no context: it runs on CuPy's current device and default stream,
+ * MPS_utils.py:15-19.) */
 int bsmps_create(int device, void *stream, bsmps_ctx **out);
 int bsmps_destroy(bsmps_ctx *ctx);
 int bsmps_synchronize(bsmps_ctx *ctx);
 const char *bsmps_last_error(void);
 int bsmps_version(void);
-/* Launch tuning: hafnians of size <= 2*lane_max_m use one lane per matrix (lane_max_m in
- * 0..5); larger ones use `group_lanes` (16, 32 or 64) cooperating lanes per matrix. */
+/* Launch tuning (no reference counterpart; set by bosonsupremacy_amd/autotune.py):
+ * hafnians of size <= 2*lane_max_m use one lane per matrix (lane_max_m in 0..5; sizes <= 4
+ * always do); larger ones use `group_lanes` (16, 32 or 64) cooperating lanes per matrix,
+ * for every size (bsmps_set_tuning) or for size n only (bsmps_set_group_lanes). */
 int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
+int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
 
 /* ---- hafnians (L3) --------------------------------------------------------------------- */
 
@@ -162,6 +167,20 @@ int bsmps_sample_site(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, const d
                       const double *eigvec, const double *eigval, const double *lambda,
                       const double *uniforms, int32_t *samples, int64_t samples_stride,
                       double *pre, double *probs);
+
+/* Reduced-precision form of the two calls above (no reference counterpart; the reference's
+ * only precision knob is complex64 throughout, MPS_cpu.py:17): the contraction runs on FP32
+ * MFMA (v_mfma_f32_16x16x4_f32) with FP32 accumulation over complex64 pre / gamma / T, and the
+ * draw reads complex64 T, computes D(alpha) and the probabilities in FP64, and writes the
+ * carried vector as complex64 rescaled by its largest component (as the reference's GPU path
+ * does, distributed_sampling.py:147), which leaves the normalised probabilities unchanged. */
+int bsmps_site_contract_c64(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const float *pre,
+                            const double *lambda_prev, const float *gamma, float *T);
+int bsmps_sample_site_c64(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, const float *T,
+                          int64_t T_sample_stride, const double *alpha, int64_t alpha_stride,
+                          const double *eigvec, const double *eigval, const double *lambda,
+                          const double *uniforms, int32_t *samples, int64_t samples_stride, float *pre,
+                          double *probs);
 
 /* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
  * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:

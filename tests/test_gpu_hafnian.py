@@ -8,7 +8,7 @@ from oracle import c_oracle
 
 pytestmark = pytest.mark.gpu
 
-TUNINGS = [(5, 64), (4, 64), (4, 32), (4, 16), (2, 64)]
+TUNINGS = [(4, 64), (5, 64), (4, 32), (4, 16), (2, 16)]
 
 
 @pytest.fixture(scope="module")
@@ -91,4 +91,4 @@ def test_invalid_sizes_raise(ctx):
     with pytest.raises(_lib.BsmpsError):
         mps.A_elem(np.eye(4), np.zeros((2, 26), np.int32), np.ones(2))
     with pytest.raises(_lib.BsmpsError):
-        ctx.set_tuning(7, 64)
+        ctx.set_tuning(6, 64)

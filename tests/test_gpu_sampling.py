@@ -102,3 +102,40 @@ def test_sampling_m32_first_order_marginals():
     assert np.all(np.abs(probs.sum(axis=2) - 1) < 1e-12)
     smp = runtime.to_host(samples)
     assert smp.min() >= 0 and smp.max() < inst.dd
+
+
+def test_site_contract_c64_matches_matmul():
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(7)
+    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4)):
+        pre = (rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))).astype(np.complex64)
+        lam = rng.uniform(0.1, 1, size=chi_p)
+        G = (rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))).astype(np.complex64)
+        want = ((pre.astype(np.complex128) * lam) @ G.astype(np.complex128).reshape(chi_p, chi * d)).reshape(n, chi, d)
+        pre_d, lam_d, G_d = runtime.to_device(pre), runtime.to_device(lam), runtime.to_device(G)
+        T = torch.empty((n, chi, d), dtype=torch.complex64, device=ctx.device)
+        ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre_d), runtime.ptr(lam_d),
+                 runtime.ptr(G_d), runtime.ptr(T))
+        got = runtime.to_host(T).astype(np.complex128)
+        assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max(), (n, chi_p, chi, d)
+
+
+def test_sampling_fp32_form_m8():
+    """FP32 contraction form: conditional distributions within 1e-5 of the FP64 form on the
+    same random numbers, and the same draws away from cdf boundaries."""
+    from bosonsupremacy_amd import runtime, sampling
+    inst, G, L = _mps_m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 3000
+    rng = np.random.default_rng(4)
+    normals = runtime.to_device(rng.normal(size=(2 * inst.M, n)))
+    uniforms = runtime.to_device(rng.uniform(size=(inst.M, n)))
+    s64 = sampling.Sampler(G, L, sqrtW, inst.dd)
+    s32 = sampling.Sampler(G, L, sqrtW, inst.dd, precision="fp32")
+    a, pa = s64.run(normals, uniforms, return_probs=True)
+    b, pb = s32.run(normals, uniforms, return_probs=True)
+    a, b, pa, pb = (runtime.to_host(x) for x in (a, b, pa, pb))
+    same = np.all(a == b, axis=1)
+    assert same.mean() > 0.99
+    assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-5
