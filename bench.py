@@ -253,7 +253,7 @@ def main():
     forms = {}
     normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=2)
     rb = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         s = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
         s.run(normals, uniforms)
         torch.cuda.synchronize()
@@ -265,9 +265,10 @@ def main():
         rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
         forms[prec] = {"samples_per_s": n_samples / dt}
         del probs
-    rel = torch.abs(rb["fp32"] - rb["fp64"]) / torch.abs(rb["fp64"])
-    forms["fp32"]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
-    forms["fp32"]["speedup_vs_fp64"] = forms["fp32"]["samples_per_s"] / forms["fp64"]["samples_per_s"]
+    for prec in ("fp32", "bf16x3"):
+        rel = torch.abs(rb[prec] - rb["fp64"]) / torch.abs(rb["fp64"])
+        forms[prec]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
+        forms[prec]["speedup_vs_fp64"] = forms[prec]["samples_per_s"] / forms["fp64"]["samples_per_s"]
 
     cpu = None
     if not args.no_cpu_baseline:

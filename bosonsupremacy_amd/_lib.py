@@ -69,6 +69,9 @@ _PROTOTYPES = {
     "bsmps_sample_site_c64": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
                               c_int),
+    "bsmps_split_bf16": ([c_void_p, c_void_p, c_int64, c_void_p, c_void_p], c_int),
+    "bsmps_site_contract_bf16x3": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p], c_int),
     "bsmps_philox": ([c_void_p, c_uint64, c_uint32, c_int, c_int64, c_int64, c_int, c_void_p], c_int),
 }
 

@@ -9,6 +9,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -88,155 +89,291 @@ __global__ void batch_displaces_kernel(int dd, const double* __restrict__ alpha,
 }
 
 // ---------------------------------------------------------------------------------------
-// Site contraction as a real FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64):
+// Site contraction as a real GEMM on MFMA:
 //   C[s][2c + {0,1}] = sum_{b} A[s][2b + {0,1}] * Bx[2b + {0,1}][2c + {0,1}]
 // with A[s][2b] = Re(pre) lambda, A[s][2b+1] = Im(pre) lambda and Bx the 2x2 real block
 // [[Re g, Im g], [-Im g, Re g]] of each complex Gamma entry g -- so C is the complex product
-// stored interleaved, exactly T's layout.  M = n, N = 2 chi d, K = 2 chi_prev.
-// Workgroup tile 64 x 64 x 16, four waves of 32 x 32 (2 x 2 MFMA blocks), LDS staged.
+// stored interleaved, exactly T's layout.  M = n, N = 2 chi d, K = 2 chi_prev; 8 n chi^2 d
+// real flops.  Workgroup tile 128 x 128 x 16 (256 threads, four waves of 64 x 64 = 4 x 4 MFMA
+// 16x16 blocks); the next K tile is fetched into registers while the current one computes.
 // ---------------------------------------------------------------------------------------
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 128, TN = 128, TK = 16;
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
+template <typename E>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<double> {
+    typedef double4_t acc_t;
+    typedef double2 pair_t;
+    __device__ static acc_t mfma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // register r of lane l holds D[4 r + l/16][l % 16]
+    __device__ static int row(int lane, int r) { return 4 * r + lane / 16; }
+};
+template <>
+struct MfmaTraits<float> {
+    typedef float4_t acc_t;
+    typedef float2 pair_t;
+    __device__ static acc_t mfma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // register r of lane l holds D[4 (l/16) + r][l % 16]
+    __device__ static int row(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
+template <typename E>
 __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
-                                                        const double* __restrict__ pre,
-                                                        const double* __restrict__ lam,
-                                                        const double* __restrict__ gamma, double* __restrict__ T) {
-    __shared__ double As[TK][TM + 1];  // As[k][row]
-    __shared__ double Bs[TK][TN + 1];  // Bs[k][col]
+                                                        const E* __restrict__ pre, const double* __restrict__ lam,
+                                                        const E* __restrict__ gamma, E* __restrict__ T) {
+    typedef MfmaTraits<E> Tr;
+    typedef typename Tr::acc_t acc_t;
+    typedef typename Tr::pair_t E2;
+    __shared__ E As[TK][TM + 2];  // As[k][row]
+    __shared__ E Bs[TK][TN + 2];  // Bs[k][col]
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int64_t row0 = (int64_t)blockIdx.y * TM;
     const int64_t col0 = (int64_t)blockIdx.x * TN;  // real column
     const int64_t Nreal = 2 * (int64_t)ncols;
     const int Kreal = 2 * chi_prev;
-    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
 
-    double4_t acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < Kreal; k0 += TK) {
-        // A tile: 64 rows x 8 complex b = 512 complex, 2 per thread
-        for (int e = tid; e < TM * (TK / 2); e += 256) {
+    // per thread: 4 complex of the A tile (128 rows x 8 b) and 2 of the B tile (8 b x 64 c)
+    E2 ra[4], rb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
             const int r = e / (TK / 2), bb = e % (TK / 2);
-            const int64_t s = row0 + r;
+            const int64_t srow = row0 + r;
             const int b = k0 / 2 + bb;
-            double re = 0.0, im = 0.0;
-            if (s < n && b < chi_prev) {
-                const cd p = ld_cd(pre, s * chi_prev + b);
+            E2 v;
+            v.x = 0;
+            v.y = 0;
+            if (srow < n && b < chi_prev) {
+                v = reinterpret_cast<const E2*>(pre)[srow * chi_prev + b];
                 const double l = lam[b];
-                re = p.re * l;
-                im = p.im * l;
+                v.x = (E)(v.x * l);
+                v.y = (E)(v.y * l);
             }
-            As[2 * bb][r] = re;
-            As[2 * bb + 1][r] = im;
+            ra[q] = v;
         }
-        // B tile: 8 complex b x 32 complex c = 256 complex, 1 per thread, expanded 2 x 2
-        {
-            const int bb = tid / (TN / 2), cc = tid % (TN / 2);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / (TN / 2), cc = e % (TN / 2);
             const int b = k0 / 2 + bb;
             const int64_t c = col0 / 2 + cc;
-            double gr = 0.0, gi = 0.0;
-            if (b < chi_prev && c < ncols) {
-                const cd g = ld_cd(gamma, (int64_t)b * ncols + c);
-                gr = g.re;
-                gi = g.im;
-            }
-            Bs[2 * bb][2 * cc] = gr;
-            Bs[2 * bb][2 * cc + 1] = gi;
-            Bs[2 * bb + 1][2 * cc] = -gi;
-            Bs[2 * bb + 1][2 * cc + 1] = gr;
+            E2 g;
+            g.x = 0;
+            g.y = 0;
+            if (b < chi_prev && c < ncols) g = reinterpret_cast<const E2*>(gamma)[(int64_t)b * ncols + c];
+            rb[q] = g;
         }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (TK / 2), bb = e % (TK / 2);
+            As[2 * bb][r] = ra[q].x;
+            As[2 * bb + 1][r] = ra[q].y;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / (TN / 2), cc = e % (TN / 2);
+            Bs[2 * bb][2 * cc] = rb[q].x;
+            Bs[2 * bb][2 * cc + 1] = rb[q].y;
+            Bs[2 * bb + 1][2 * cc] = -rb[q].y;
+            Bs[2 * bb + 1][2 * cc + 1] = rb[q].x;
+        }
+    };
+
+    fetch(0);
+    for (int k0 = 0; k0 < Kreal; k0 += TK) {
+        stash();
         __syncthreads();
+        if (k0 + TK < Kreal) fetch(k0 + TK);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
-            double a[2], b[2];
-            // operand layout of v_mfma_f64_16x16x4f64: lane holds A[lane%16][lane/16], B[lane/16][lane%16]
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + lane / 16][wm + i * 16 + lane % 16];
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + lane / 16][wn + j * 16 + lane % 16];
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            E a[4], b[4];
+            // operand layout: lane holds A[lane%16][lane/16] and B[lane/16][lane%16]
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk + lane / 16][wm + i * 16 + lane % 16];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk + lane / 16][wn + j * 16 + lane % 16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Tr::mfma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
-    // result layout of v_mfma_f64_16x16x4f64: register r of lane l holds D[4*r + l/16][l%16]
-    // (the f32 variant below differs: D[4*(l/16) + r][l%16])
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t s = row0 + wm + i * 16 + 4 * r + lane / 16;
+                const int64_t srow = row0 + wm + i * 16 + Tr::row(lane, r);
                 const int64_t c = col0 + wn + j * 16 + lane % 16;
-                if (s < n && c < Nreal) T[s * Nreal + c] = acc[i][j][r];
+                if (srow < n && c < Nreal) T[srow * Nreal + c] = acc[i][j][r];
             }
 }
 
-// FP32 form of the same contraction (v_mfma_f32_16x16x4_f32, FP32 accumulation): complex64
-// operands and output, lambda applied in FP64 then rounded.  Same tiling and lane layout.
-typedef float float4_t __attribute__((ext_vector_type(4)));
+// ---------------------------------------------------------------------------------------
+// BF16x3 split-precision form of the contraction (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation).  Every FP32 operand x is split as x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi); a product is taken as hi*hi + hi*lo + lo*hi (the lo*lo term, ~2^-16
+// relative, is dropped).  Gamma is split once per site (bsmps_split_bf16); the carried
+// vectors are split as they are staged.  Tile 128 x 128 x 32 (real), LDS rows k-contiguous
+// so each lane's 8 consecutive k values are one 16-byte read.
+// ---------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int BK = 32;             // real K per tile (16 complex b)
+constexpr int BKP = BK + 8;        // padded row length (80 bytes)
 
-__global__ __launch_bounds__(256) void site_gemm_f32_kernel(int64_t n, int chi_prev, int ncols,
-                                                            const float* __restrict__ pre,
-                                                            const double* __restrict__ lam,
-                                                            const float* __restrict__ gamma, float* __restrict__ T) {
-    __shared__ float As[TK][TM + 4];
-    __shared__ float Bs[TK][TN + 4];
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+__global__ void split_bf16_kernel(const float* __restrict__ g, int64_t count, __bf16* __restrict__ hi,
+                                  __bf16* __restrict__ lo) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        split_bf16(g[i], hi[i], lo[i]);
+}
+
+__global__ __launch_bounds__(256) void site_gemm_bf16x3_kernel(int64_t n, int chi_prev, int ncols,
+                                                               const float* __restrict__ pre,
+                                                               const double* __restrict__ lam,
+                                                               const __bf16* __restrict__ ghi,
+                                                               const __bf16* __restrict__ glo, float* __restrict__ T) {
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TM][BKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TM][BKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[TN][BKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[TN][BKP];
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int64_t row0 = (int64_t)blockIdx.y * TM;
     const int64_t col0 = (int64_t)blockIdx.x * TN;
     const int64_t Nreal = 2 * (int64_t)ncols;
     const int Kreal = 2 * chi_prev;
-    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
-    float4_t acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < Kreal; k0 += TK) {
-        for (int e = tid; e < TM * (TK / 2); e += 256) {
-            const int r = e / (TK / 2), bb = e % (TK / 2);
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    float4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    // per thread: 8 complex of A (128 rows x 16 b) and 4 of B (16 b x 64 c)
+    float2 ra[8];
+    uint32_t rbh[4], rbl[4];  // (re, im) bf16 pairs
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BK / 2), bb = e % (BK / 2);
             const int64_t srow = row0 + r;
             const int b = k0 / 2 + bb;
-            float re = 0.f, im = 0.f;
+            float2 v = make_float2(0.f, 0.f);
             if (srow < n && b < chi_prev) {
-                const float2 p = reinterpret_cast<const float2*>(pre)[srow * chi_prev + b];
+                v = reinterpret_cast<const float2*>(pre)[srow * chi_prev + b];
                 const double l = lam[b];
-                re = (float)(p.x * l);
-                im = (float)(p.y * l);
+                v.x = (float)(v.x * l);
+                v.y = (float)(v.y * l);
             }
-            As[2 * bb][r] = re;
-            As[2 * bb + 1][r] = im;
+            ra[q] = v;
         }
-        {
-            const int bb = tid / (TN / 2), cc = tid % (TN / 2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / (TN / 2), cc = e % (TN / 2);
             const int b = k0 / 2 + bb;
             const int64_t c = col0 / 2 + cc;
-            float gr = 0.f, gi = 0.f;
+            uint32_t h = 0, l = 0;
             if (b < chi_prev && c < ncols) {
-                const float2 g = reinterpret_cast<const float2*>(gamma)[(int64_t)b * ncols + c];
-                gr = g.x;
-                gi = g.y;
+                h = reinterpret_cast<const uint32_t*>(ghi)[(int64_t)b * ncols + c];
+                l = reinterpret_cast<const uint32_t*>(glo)[(int64_t)b * ncols + c];
             }
-            Bs[2 * bb][2 * cc] = gr;
-            Bs[2 * bb][2 * cc + 1] = gi;
-            Bs[2 * bb + 1][2 * cc] = -gi;
-            Bs[2 * bb + 1][2 * cc + 1] = gr;
+            rbh[q] = h;
+            rbl[q] = l;
         }
-        __syncthreads();
+    };
+    auto neg = [](uint32_t pair_hi16) -> __bf16 {  // -(bf16 stored in the high half)
+        const unsigned short u = (unsigned short)(pair_hi16 >> 16) ^ 0x8000u;
+        return __builtin_bit_cast(__bf16, u);
+    };
+    auto lo16 = [](uint32_t p) -> __bf16 { return __builtin_bit_cast(__bf16, (unsigned short)(p & 0xffffu)); };
+    auto hi16 = [](uint32_t p) -> __bf16 { return __builtin_bit_cast(__bf16, (unsigned short)(p >> 16)); };
+    auto stash = [&]() {
 #pragma unroll
-        for (int kk = 0; kk < TK; kk += 4) {
-            float a[2], b[2];
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + lane / 16][wm + i * 16 + lane % 16];
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + lane / 16][wn + j * 16 + lane % 16];
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / (BK / 2), bb = e % (BK / 2);
+            __bf16 h, l;
+            split_bf16(ra[q].x, h, l);
+            Ah[r][2 * bb] = h;
+            Al[r][2 * bb] = l;
+            split_bf16(ra[q].y, h, l);
+            Ah[r][2 * bb + 1] = h;
+            Al[r][2 * bb + 1] = l;
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / (TN / 2), cc = e % (TN / 2);
+            // B_exp rows (2bb, 2bb+1), cols (2cc, 2cc+1): [[gr, gi], [-gi, gr]], stored [col][k]
+            Bh[2 * cc][2 * bb] = lo16(rbh[q]);
+            Bh[2 * cc + 1][2 * bb] = hi16(rbh[q]);
+            Bh[2 * cc][2 * bb + 1] = neg(rbh[q]);
+            Bh[2 * cc + 1][2 * bb + 1] = lo16(rbh[q]);
+            Bl[2 * cc][2 * bb] = lo16(rbl[q]);
+            Bl[2 * cc + 1][2 * bb] = hi16(rbl[q]);
+            Bl[2 * cc][2 * bb + 1] = neg(rbl[q]);
+            Bl[2 * cc + 1][2 * bb + 1] = lo16(rbl[q]);
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < Kreal; k0 += BK) {
+        stash();
+        __syncthreads();
+        if (k0 + BK < Kreal) fetch(k0 + BK);
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+        const int ko = 8 * (lane / 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wm + i * 16 + lane % 16][ko]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[wm + i * 16 + lane % 16][ko]);
+            bh[i] = *reinterpret_cast<const bf16x8*>(&Bh[wn + i * 16 + lane % 16][ko]);
+            bl[i] = *reinterpret_cast<const bf16x8*>(&Bl[wn + i * 16 + lane % 16][ko]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
         __syncthreads();
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t srow = row0 + wm + i * 16 + 4 * (lane / 16) + r;
                 const int64_t c = col0 + wn + j * 16 + lane % 16;
@@ -505,8 +642,8 @@ extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int 
         bsmps_set_error("bsmps_site_contract: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n, chi_prev,
-                       (int)ncols, pre, lambda_prev, gamma, T);
+    hipLaunchKernelGGL(site_gemm_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                       chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
@@ -572,7 +709,7 @@ extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, 
         bsmps_set_error("bsmps_site_contract_c64: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_f32_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+    hipLaunchKernelGGL(site_gemm_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
                        chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
@@ -601,6 +738,47 @@ extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, 
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
 #undef BSMPS_SAMPLE_LAUNCH
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_split_bf16(bsmps_ctx* ctx, const float* x, int64_t count, void* hi, void* lo) {
+    if (!ctx || count < 0) {
+        bsmps_set_error("bsmps_split_bf16: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (count == 0) return BSMPS_OK;
+    if (!x || !hi || !lo) {
+        bsmps_set_error("bsmps_split_bf16: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid_for(count, 256)), dim3(256), 0, ctx->stream, x, count,
+                       reinterpret_cast<__bf16*>(hi), reinterpret_cast<__bf16*>(lo));
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract_bf16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const float* pre,
+                                          const double* lambda_prev, const void* gamma_hi, const void* gamma_lo,
+                                          float* T) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_site_contract_bf16x3: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre || !lambda_prev || !gamma_hi || !gamma_lo || !T) {
+        bsmps_set_error("bsmps_site_contract_bf16x3: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d;
+    const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
+    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+        bsmps_set_error("bsmps_site_contract_bf16x3: problem too large for one launch");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(site_gemm_bf16x3_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                       chi_prev, (int)ncols, pre, lambda_prev, reinterpret_cast<const __bf16*>(gamma_hi),
+                       reinterpret_cast<const __bf16*>(gamma_lo), T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }

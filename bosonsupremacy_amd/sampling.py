@@ -98,7 +98,7 @@ class Sampler:
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
     """
 
-    PRECISIONS = ("fp64", "fp32")
+    PRECISIONS = ("fp64", "fp32", "bf16x3")
 
     def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64"):
         self.ctx = runtime.context()
@@ -113,6 +113,16 @@ class Sampler:
         self.precision = precision
         ctype = torch.complex128 if precision == "fp64" else torch.complex64
         self.gammas = [g.to(ctype).contiguous() for g in gammas]
+        self.splits = None
+        if precision == "bf16x3":
+            # hi / lo bf16 planes of every Gamma, split once and reused by every batch
+            self.splits = []
+            for g in self.gammas:
+                cnt = 2 * g.numel()
+                hi = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+                lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+                self.ctx.call("bsmps_split_bf16", runtime.ptr(g), cnt, runtime.ptr(hi), runtime.ptr(lo))
+                self.splits.append((hi, lo))
         self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
         self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
         W, x = displacement_basis(dd)
@@ -134,8 +144,7 @@ class Sampler:
         ctx.call("bsmps_mu_to_alpha", runtime.ptr(self.sqrtW), M, runtime.ptr(normals), n, runtime.ptr(alpha))
         samples = torch.empty((n, M), dtype=torch.int32, device=self.dev)
         ctype = torch.complex128 if self.precision == "fp64" else torch.complex64
-        contract, draw = (("bsmps_site_contract", "bsmps_sample_site") if self.precision == "fp64"
-                          else ("bsmps_site_contract_c64", "bsmps_sample_site_c64"))
+        draw = "bsmps_sample_site" if self.precision == "fp64" else "bsmps_sample_site_c64"
         pre = torch.empty((n, chi), dtype=ctype, device=self.dev)
         T = torch.empty((n, chi, d), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
@@ -145,8 +154,16 @@ class Sampler:
                 src, stride = self.gammas[0], 0  # Gamma_0[0, a, j] is the site-0 tensor for every sample
             else:
                 with timer("site_contract"):
-                    ctx.call(contract, n, chi, chi, d, runtime.ptr(pre),
-                             runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                    if self.precision == "fp64":
+                        ctx.call("bsmps_site_contract", n, chi, chi, d, runtime.ptr(pre),
+                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                    elif self.precision == "fp32":
+                        ctx.call("bsmps_site_contract_c64", n, chi, chi, d, runtime.ptr(pre),
+                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                    else:
+                        hi, lo = self.splits[i]
+                        ctx.call("bsmps_site_contract_bf16x3", n, chi, chi, d, runtime.ptr(pre),
+                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(hi), runtime.ptr(lo), runtime.ptr(T))
                 src, stride = T, chi * d
             a_ptr = ctypes.c_void_p(alpha.data_ptr() + 16 * i)
             s_ptr = ctypes.c_void_p(samples.data_ptr() + 4 * i)

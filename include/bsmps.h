@@ -182,6 +182,15 @@ int bsmps_sample_site_c64(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, con
                           const double *uniforms, int32_t *samples, int64_t samples_stride, float *pre,
                           double *probs);
 
+/* BF16x3 split-precision form of the contraction (no reference counterpart): each FP32
+ * operand x is split into bf16 hi = bf16(x) and lo = bf16(x - hi); products are
+ * hi*hi + hi*lo + lo*hi on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
+ * gamma_hi / gamma_lo come from bsmps_split_bf16 applied to gamma's complex64 storage
+ * (2*chi_prev*chi*d floats); pre and T are complex64 as in bsmps_site_contract_c64. */
+int bsmps_split_bf16(bsmps_ctx *ctx, const float *x, int64_t count, void *hi, void *lo);
+int bsmps_site_contract_bf16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const float *pre,
+                               const double *lambda_prev, const void *gamma_hi, const void *gamma_lo, float *T);
+
 /* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
  * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:
  * counter (e_lo, e_hi, stream, 0) with e = first+i, key (seed_lo, seed_hi).  A uniform is

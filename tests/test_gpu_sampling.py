@@ -139,3 +139,41 @@ def test_sampling_fp32_form_m8():
     same = np.all(a == b, axis=1)
     assert same.mean() > 0.99
     assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-5
+
+
+def test_site_contract_bf16x3_matches_matmul():
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(8)
+    for n, chi_p, chi, d in ((70, 33, 17, 3), (300, 64, 64, 4)):
+        pre = (rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))).astype(np.complex64)
+        lam = rng.uniform(0.1, 1, size=chi_p)
+        G = (rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))).astype(np.complex64)
+        want = ((pre.astype(np.complex128) * lam) @ G.astype(np.complex128).reshape(chi_p, chi * d)).reshape(n, chi, d)
+        pre_d, lam_d, G_d = runtime.to_device(pre), runtime.to_device(lam), runtime.to_device(G)
+        hi = torch.empty((2 * G.size,), dtype=torch.bfloat16, device=ctx.device)
+        lo = torch.empty((2 * G.size,), dtype=torch.bfloat16, device=ctx.device)
+        ctx.call("bsmps_split_bf16", runtime.ptr(G_d), 2 * G.size, runtime.ptr(hi), runtime.ptr(lo))
+        T = torch.empty((n, chi, d), dtype=torch.complex64, device=ctx.device)
+        ctx.call("bsmps_site_contract_bf16x3", n, chi_p, chi, d, runtime.ptr(pre_d), runtime.ptr(lam_d),
+                 runtime.ptr(hi), runtime.ptr(lo), runtime.ptr(T))
+        got = runtime.to_host(T).astype(np.complex128)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("bf16x3 rel err", err, flush=True)
+        assert err <= 2e-4, (n, chi_p, chi, d)
+
+
+def test_sampling_bf16x3_form_m8():
+    from bosonsupremacy_amd import runtime, sampling
+    inst, G, L = _mps_m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 2000
+    rng = np.random.default_rng(5)
+    normals = runtime.to_device(rng.normal(size=(2 * inst.M, n)))
+    uniforms = runtime.to_device(rng.uniform(size=(inst.M, n)))
+    a, pa = sampling.Sampler(G, L, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
+    b, pb = sampling.Sampler(G, L, sqrtW, inst.dd, precision="bf16x3").run(normals, uniforms, return_probs=True)
+    a, b, pa, pb = (runtime.to_host(x) for x in (a, b, pa, pb))
+    same = np.all(a == b, axis=1)
+    assert same.mean() > 0.98
+    assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-3
