@@ -20,6 +20,7 @@ SITE_FIRST, SITE_MIDDLE, SITE_LAST = 0, 1, 2
 c_int, c_int64, c_uint64, c_uint32, c_double = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint64,
                                                ctypes.c_uint32, ctypes.c_double)
 c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
 
@@ -52,6 +53,8 @@ _PROTOTYPES = {
     "bsmps_version": ([], c_int),
     "bsmps_set_tuning": ([c_void_p, c_int, c_int], c_int),
     "bsmps_set_group_lanes": ([c_void_p, c_int, c_int], c_int),
+    "bsmps_set_split": ([c_void_p, c_int, c_int], c_int),
+    "bsmps_set_split_budget": ([c_void_p, c_size_t], c_int),
     "bsmps_sigma_select": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_dense": ([c_void_p, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p], c_int),

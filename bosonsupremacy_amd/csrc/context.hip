@@ -36,6 +36,21 @@ int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out) {
     return BSMPS_OK;
 }
 
+int bsmps_ctx_scratch(bsmps_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->hws_bytes) {
+        if (ctx->hws) {
+            BSMPS_HIP(hipStreamSynchronize(ctx->stream));
+            BSMPS_HIP(hipFree(ctx->hws));
+            ctx->hws = nullptr;
+            ctx->hws_bytes = 0;
+        }
+        BSMPS_HIP(hipMalloc(&ctx->hws, bytes));
+        ctx->hws_bytes = bytes;
+    }
+    *out = ctx->hws;
+    return BSMPS_OK;
+}
+
 extern "C" const char* bsmps_last_error(void) { return g_last_error; }
 
 extern "C" int bsmps_version(void) { return 1; }
@@ -59,6 +74,10 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     c->ws_bytes = 0;
     c->lane_max_m = 4;
     for (int m = 0; m < 16; ++m) c->group_lanes_by_m[m] = 64;
+    for (int m = 0; m < 16; ++m) c->split_by_m[m] = (m >= 6 && m <= 10) ? 1 : 0;
+    c->hws = nullptr;
+    c->hws_bytes = 0;
+    c->split_chunk_bytes = (size_t)1 << 30;
     c->max_lds = 64 * 1024;
     {
         int v = 0;
@@ -78,6 +97,7 @@ extern "C" int bsmps_destroy(bsmps_ctx* ctx) {
     int st = BSMPS_OK;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = BSMPS_ERR_HIP;
     if (ctx->ws) hipFree(ctx->ws);
+    if (ctx->hws) hipFree(ctx->hws);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return st;
@@ -111,5 +131,23 @@ extern "C" int bsmps_set_group_lanes(bsmps_ctx* ctx, int n, int group_lanes) {
         return BSMPS_ERR_INVALID;
     }
     ctx->group_lanes_by_m[n / 2] = group_lanes;
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_set_split(bsmps_ctx* ctx, int n, int enable) {
+    if (!ctx || n < 12 || n > 20 || n % 2 || (enable != 0 && enable != 1)) {
+        bsmps_set_error("bsmps_set_split: n must be even in 12..20 and enable 0 or 1");
+        return BSMPS_ERR_INVALID;
+    }
+    ctx->split_by_m[n / 2] = enable;
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_set_split_budget(bsmps_ctx* ctx, size_t bytes) {
+    if (!ctx || bytes < ((size_t)1 << 20)) {
+        bsmps_set_error("bsmps_set_split_budget: the budget must be at least 1 MiB");
+        return BSMPS_ERR_INVALID;
+    }
+    ctx->split_chunk_bytes = bytes;
     return BSMPS_OK;
 }

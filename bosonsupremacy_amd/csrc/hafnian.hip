@@ -444,9 +444,21 @@ __device__ __forceinline__ void child_entry_dispatch(int D, const cd* Ar, const 
     }
 }
 
-template <int M, class S>
+// Where the split path's upper phase writes the frontier subtrees' inputs: complex slot `i` of
+// hafnian `h` (0 <= h < n_h, relative to the chunk's first item) lives at fdata[i * stride + h],
+// hafnian index fastest, so the one-lane-per-subtree kernels read it coalesced.
+struct SplitOut {
+    double* fdata;
+    int64_t stride;
+    int64_t n_h;
+    const int4* front;  // per frontier node: {k | D << 8 | G << 16, owner | g_level << 8, offset, w > 0}
+    int abot_k;         // the bottom block holds A(J, K), J > K >= 2 abot_k, at slots abot_off..
+    int abot_off;
+};
+
+template <int M, class S, bool SPLIT>
 __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, Layout lay,
-                                                          const int4* __restrict__ ops, int G) {
+                                                          const int4* __restrict__ ops, int G, SplitOut so) {
     extern __shared__ cd smem[];
     const int PER = kWave / G;  // matrices per workgroup
     const int grp = threadIdx.x / G;
@@ -455,9 +467,10 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
     const int n = 2 * m;
     cd* R = smem + grp * lay.total;
     int* tg = reinterpret_cast<int*>(R + lay.off_tgt);
-    const int64_t item = item0 + (int64_t)blockIdx.x * PER + grp;
-    const bool active = item < src.n_items;
-    const int64_t eff = active ? item : src.n_items - 1;
+    const int64_t hl = (int64_t)blockIdx.x * PER + grp;  // position within this launch
+    const int64_t item = item0 + hl;
+    const bool active = SPLIT ? hl < so.n_h : item < src.n_items;
+    const int64_t eff = active ? item : (SPLIT ? item0 + so.n_h - 1 : src.n_items - 1);
 
     typename std::conditional<std::is_same<S, GammaSource>::value, GammaSource::Decoded, int>::type dc{};
     if constexpr (std::is_same<S, GammaSource>::value) dc = src.decode(eff);
@@ -477,12 +490,46 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
     }
     if (gl <= m) R[lay.off_g + gl] = gl == 0 ? cd{1.0, 0.0} : czero();  // g at level 0
     __syncthreads();
+    if constexpr (SPLIT) {  // bottom block of the gathered matrix, read by every subtree
+        const int s0 = n - 2 * so.abot_k;
+        if (active)
+            for (int i = gl; i < s0 * (s0 - 1) / 2; i += G) {
+                int Jl, Kl;
+                decode_pair(i, Jl, Kl);
+                st_cd(so.fdata, (so.abot_off + i) * so.stride + hl,
+                      R[lay.off_A + tri(Jl + 2 * so.abot_k, Kl + 2 * so.abot_k)]);
+            }
+    }
 
     int sp = 0;  // value-stack depth (uniform)
     for (int pc = 0;; ++pc) {
         const int4 op = ops[pc];
         const int kind = op.x & 0xff;
         if (kind == 3) break;
+        if (SPLIT && kind == 4) {  // EMIT: copy a frontier node's polynomials and g out
+            const int4 fr = so.front[op.y];
+            const int k = fr.x & 0xff, D = (fr.x >> 8) & 0xff, Gd = (fr.x >> 16) & 0xff;
+            const int owner = fr.y & 0xff, glev = (fr.y >> 8) & 0xff;
+            const int s0 = n - 2 * k;
+            const int n_p = s0 * (s0 - 1) / 2 * D;
+            const cd* So = R + lay.off_S[owner > 0 ? owner : 1];
+            const int ocap = owner > 0 ? lay.cap[owner] : 0;
+            if (active)
+                for (int i = gl; i < n_p + Gd + 1; i += G) {
+                    cd v;
+                    if (i < n_p) {
+                        const int e = i / D, t = i - e * D;  // coefficient t + 1 of entry e
+                        int Jl, Kl;
+                        decode_pair(e, Jl, Kl);
+                        v = So[tri(Jl + 2 * (k - owner), Kl + 2 * (k - owner)) * ocap + t];
+                    } else {
+                        v = R[lay.off_g + glev * (m + 1) + (i - n_p)];
+                    }
+                    st_cd(so.fdata, (fr.z + i) * so.stride + hl, v);
+                }
+            __syncthreads();
+            continue;
+        }
         if (kind == 1) {  // LEAF: push w * g[m]
             if (gl == 0) {
                 const int gs = op.z & 0xff;
@@ -536,11 +583,88 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
         }
     }
     __syncthreads();
-    if (active && gl == 0) {
+    if (!SPLIT && active && gl == 0) {
         const cd h = R[lay.off_val];
         if constexpr (std::is_same<S, GammaSource>::value) src.store_dec(dc, h);
         else src.store(item, h);
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// Frontier split, lower phase: one lane per (hafnian, frontier subtree).  A subtree rooted at
+// level K0 with node degree D (and g degree D: every frontier node the plan makes has G == D)
+// is evaluated by the same compile-time recursion as the lane path, from the polynomials and
+// g the upper phase wrote out; its value goes to vals[f * stride + h].
+// ---------------------------------------------------------------------------------------
+
+template <int OFF>
+struct LdsAOff {  // A(J, K) of a subtree whose rows start at OFF, [entry][lane] in LDS
+    const cd* base;
+    __device__ __forceinline__ cd operator()(int J, int K) const { return base[tri(J - OFF, K - OFF) * kWave]; }
+};
+
+template <int M, int K0, int D>
+__global__ __launch_bounds__(kWave) void haf_subtree_kernel(const double* __restrict__ fdata, int64_t stride,
+                                                            int64_t n_h, const int4* __restrict__ front,
+                                                            const int* __restrict__ tlist, int abot_k,
+                                                            int abot_off, double* __restrict__ vals) {
+    constexpr int S0 = 2 * (M - K0);
+    constexpr int NA = S0 * (S0 - 1) / 2;
+    __shared__ cd lds[NA * kWave];
+    const int lane = threadIdx.x;
+    const int64_t h = (int64_t)blockIdx.x * kWave + lane;
+    const int f = tlist[blockIdx.y];
+    const int4 fr = front[f];
+    if (h >= n_h) return;
+    const int ak = 2 * abot_k;
+#pragma unroll
+    for (int J = 2 * K0 + 1; J < 2 * M; ++J)
+#pragma unroll
+        for (int K = 2 * K0; K < J; ++K)
+            lds[tri(J - 2 * K0, K - 2 * K0) * kWave + lane] = ld_cd(fdata, (abot_off + tri(J - ak, K - ak)) * stride + h);
+    LdsAOff<2 * K0> A{lds + lane};
+    cd P[LaneDims<M>::NE][LaneDims<M>::NC];
+    const int64_t base = fr.z;
+#pragma unroll
+    for (int J = 2 * K0 + 1; J < 2 * M; ++J)
+#pragma unroll
+        for (int K = 2 * K0; K < J; ++K)
+#pragma unroll
+            for (int t = 0; t < D; ++t)
+                P[tri(J, K)][t] = ld_cd(fdata, (base + tri(J - 2 * K0, K - 2 * K0) * D + t) * stride + h);
+    cd g[M + 1];
+#pragma unroll
+    for (int t = 0; t <= M; ++t) g[t] = t <= D ? ld_cd(fdata, (base + NA * D + t) * stride + h) : czero();
+    const double w = fr.w ? 1.0 : -1.0;
+    st_cd(vals, (int64_t)f * stride + h, Node<M, K0, D, D>::run(A, P, g, w));
+}
+
+// Frontier split, last phase: add the subtree values in the order of the recursion tree
+// (comb: f >= 0 pushes subtree f, -1 adds the top two) and apply the item's epilogue.
+constexpr int kCombineStack = 18;
+template <class S>
+__global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0, int64_t n_h, int64_t stride,
+                                                            const double* __restrict__ vals,
+                                                            const int* __restrict__ comb, int n_comb) {
+    __shared__ cd stack[kCombineStack * kWave];
+    const int lane = threadIdx.x;
+    const int64_t h = (int64_t)blockIdx.x * kWave + lane;
+    if (h >= n_h) return;
+    int sp = 0;
+    for (int i = 0; i < n_comb; ++i) {
+        const int c = comb[i];
+        if (c >= 0) {
+            stack[sp * kWave + lane] = ld_cd(vals, (int64_t)c * stride + h);
+            ++sp;
+        } else {
+            stack[(sp - 2) * kWave + lane] = cadd(stack[(sp - 2) * kWave + lane], stack[(sp - 1) * kWave + lane]);
+            --sp;
+        }
+    }
+    const cd v = stack[lane];
+    const int64_t item = item0 + h;
+    if constexpr (std::is_same<S, GammaSource>::value) src.store_dec(src.decode(item), v);
+    else src.store(item, v);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -591,12 +715,12 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
 #define BSMPS_GROUP_CASE(MM)                                                                                  \
     case MM: {                                                                                                \
         if (lds > 64 * 1024)                                                                                  \
-            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S>,                              \
+            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S, false>,                       \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
         for (int64_t b0 = 0; b0 < blocks; b0 += kMaxGrid) {                                                   \
             const int64_t nb = std::min<int64_t>(kMaxGrid, blocks - b0);                                      \
-            hipLaunchKernelGGL((haf_group_kernel<MM, S>), dim3((unsigned)nb), dim3(kWave), lds, ctx->stream, src, \
-                               b0 * per, plan->lay, plan->d_ops, G);                                          \
+            hipLaunchKernelGGL((haf_group_kernel<MM, S, false>), dim3((unsigned)nb), dim3(kWave), lds, ctx->stream, \
+                               src, b0 * per, plan->lay, plan->d_ops, G, SplitOut{});                         \
             BSMPS_HIP(hipGetLastError());                                                                     \
         }                                                                                                     \
         break;                                                                                                \
@@ -619,11 +743,208 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
     return BSMPS_OK;
 }
 
+// Frontier-split plan of one size: the upper schedule (RIGHT / EMIT / END ops), the frontier
+// nodes with their data offsets, the combine program and the frontier nodes grouped by type.
+// A node is a frontier node when its subtree fits one lane: depth <= 4 with degree <= 1,
+// depth <= 3 with degree <= 3, or depth <= 2.
+struct FrontierBuilder {
+    int m;
+    ScheduleBuilder nzb;
+    std::vector<int4> ops;
+    std::vector<int4> front;  // {k | D << 8 | G << 16, owner | g_level << 8, offset, w > 0}
+    std::vector<int> comb;
+    int slots = 0;
+    bool is_frontier(int level, int D) const {
+        const int depth = m - level;
+        return (depth <= 4 && D <= 1) || (depth <= 3 && D <= 3) || depth <= 2;
+    }
+    void gen(int level, int owner, int D, int G, int gl, int w) {
+        if (is_frontier(level, D)) {
+            const int s0 = 2 * m - 2 * level;
+            comb.push_back((int)front.size());
+            ops.push_back(make_int4(4, (int)front.size(), 0, 0));
+            front.push_back(make_int4(level | (D << 8) | (G << 16), owner | (gl << 8), slots, w > 0));
+            slots += s0 * (s0 - 1) / 2 * D + G + 1;
+            return;
+        }
+        const int D2 = std::min(m, 2 * D + 1), G2 = std::min(m, G + D + 1);
+        const bool lnz = nzb.nonzero(level + 1, D, G), rnz = nzb.nonzero(level + 1, D2, G2);
+        if (lnz) gen(level + 1, owner, D, G, gl, -w);
+        if (rnz) {
+            ops.push_back(make_int4(0 | (level << 8) | (owner << 16), D | (D2 << 8) | (G << 16) | (G2 << 24),
+                                    gl | ((level + 1) << 8) | ((w > 0) << 16), 0));
+            gen(level + 1, level + 1, D2, G2, level + 1, w);
+        }
+        if (lnz && rnz) comb.push_back(-1);
+    }
+};
+
+// Subtree shapes with a compiled one-lane kernel: (K0, D) = (M-4, 0), (M-4, 1), (M-3, 3),
+// (M-2, 7) and (M-2, M), for M = 6..10.
+constexpr int kSplitMinM = 6, kSplitMaxM = 10;
+bool subtree_compiled(int M, int K0, int D) {
+    if (M < kSplitMinM || M > kSplitMaxM) return false;
+    return (K0 == M - 4 && (D == 0 || D == 1)) || (K0 == M - 3 && D == 3) ||
+           (K0 == M - 2 && (D == M || (D == 7 && M > 7)));
+}
+
+struct FrontierPlan {
+    bool ok;
+    Layout lay;
+    int4* d_ops;
+    int4* d_front;
+    int* d_comb;
+    int n_comb;
+    int* d_tlist;
+    std::vector<int2> types;  // (K0, D)
+    std::vector<int> type_first, type_count;
+    int n_front;
+    int data_slots;  // frontier inputs + bottom block, per hafnian
+    int abot_k, abot_off;
+};
+
+std::map<std::pair<int, int>, FrontierPlan> g_fplans;  // (device, m)
+
+int get_fplan(int device, int m, FrontierPlan** out) {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto key = std::make_pair(device, m);
+    auto it = g_fplans.find(key);
+    if (it == g_fplans.end()) {
+        FrontierBuilder fb;
+        fb.m = m;
+        fb.nzb.m = m;
+        fb.gen(0, 0, 0, 0, 0, 1);
+        fb.ops.push_back(make_int4(3, 0, 0, 0));
+        FrontierPlan p{};
+        p.ok = true;
+        p.lay = make_layout(m);
+        p.n_front = (int)fb.front.size();
+        p.abot_k = m;
+        std::map<std::pair<int, int>, std::vector<int>> by_type;
+        for (int f = 0; f < p.n_front; ++f) {
+            const int4 fr = fb.front[f];
+            const int k = fr.x & 0xff, D = (fr.x >> 8) & 0xff, G = (fr.x >> 16) & 0xff;
+            if (G != D || !subtree_compiled(m, k, D)) p.ok = false;
+            p.abot_k = std::min(p.abot_k, k);
+            by_type[std::make_pair(k, D)].push_back(f);
+        }
+        const int sb = 2 * m - 2 * p.abot_k;
+        p.abot_off = fb.slots;
+        p.data_slots = fb.slots + sb * (sb - 1) / 2;
+        std::vector<int> tlist;
+        for (auto& kv : by_type) {
+            p.types.push_back(make_int2(kv.first.first, kv.first.second));
+            p.type_first.push_back((int)tlist.size());
+            p.type_count.push_back((int)kv.second.size());
+            tlist.insert(tlist.end(), kv.second.begin(), kv.second.end());
+        }
+        p.n_comb = (int)fb.comb.size();
+        int depth = 0, max_depth = 0;
+        for (int c : fb.comb) {
+            depth += c >= 0 ? 1 : -1;
+            max_depth = std::max(max_depth, depth);
+        }
+        if (max_depth > kCombineStack || depth != 1) p.ok = false;
+        BSMPS_HIP(hipMalloc(&p.d_ops, sizeof(int4) * fb.ops.size()));
+        BSMPS_HIP(hipMemcpy(p.d_ops, fb.ops.data(), sizeof(int4) * fb.ops.size(), hipMemcpyHostToDevice));
+        BSMPS_HIP(hipMalloc(&p.d_front, sizeof(int4) * fb.front.size()));
+        BSMPS_HIP(hipMemcpy(p.d_front, fb.front.data(), sizeof(int4) * fb.front.size(), hipMemcpyHostToDevice));
+        BSMPS_HIP(hipMalloc(&p.d_comb, sizeof(int) * fb.comb.size()));
+        BSMPS_HIP(hipMemcpy(p.d_comb, fb.comb.data(), sizeof(int) * fb.comb.size(), hipMemcpyHostToDevice));
+        BSMPS_HIP(hipMalloc(&p.d_tlist, sizeof(int) * tlist.size()));
+        BSMPS_HIP(hipMemcpy(p.d_tlist, tlist.data(), sizeof(int) * tlist.size(), hipMemcpyHostToDevice));
+        it = g_fplans.emplace(key, p).first;
+    }
+    *out = &it->second;
+    return BSMPS_OK;
+}
+
+template <int M>
+int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const double* fdata, int64_t stride, int64_t n_h,
+                      const int4* front, const int* tlist, int abot_k, int abot_off, double* vals) {
+#define BSMPS_SUBTREE(KK, DD)                                                                                 \
+    hipLaunchKernelGGL((haf_subtree_kernel<M, KK, DD>), grid, dim3(kWave), 0, st, fdata, stride, n_h, front,  \
+                       tlist, abot_k, abot_off, vals)
+    if (K0 == M - 4 && D == 0) BSMPS_SUBTREE(M - 4, 0);
+    else if (K0 == M - 4 && D == 1) BSMPS_SUBTREE(M - 4, 1);
+    else if (K0 == M - 3 && D == 3) BSMPS_SUBTREE(M - 3, 3);
+    else if (K0 == M - 2 && D == M) BSMPS_SUBTREE(M - 2, M);
+    else if (K0 == M - 2 && D == 7 && M > 7) BSMPS_SUBTREE(M - 2, (M > 7 ? 7 : M));
+    else {
+        bsmps_set_error("no subtree kernel for size %d, level %d, degree %d", 2 * M, K0, D);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+#undef BSMPS_SUBTREE
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+// Frontier-split launch: per chunk of hafnians, the upper phase (cooperative lanes), one launch
+// per subtree shape, then the combine; the chunk is sized so its scratch fits the budget.
+template <class S>
+int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
+    const int G = ctx->group_lanes_by_m[m];
+    const int per = kWave / G;
+    const size_t lds = sizeof(cd) * (size_t)fp->lay.total * per;
+    if (lds > (size_t)ctx->max_lds) {
+        bsmps_set_error("hafnian of size %d needs %zu bytes of LDS per workgroup with %d lanes per matrix "
+                        "(device limit %d)", 2 * m, lds, G, ctx->max_lds);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    const size_t per_h = sizeof(cd) * (size_t)(fp->data_slots + fp->n_front);
+    int64_t H = (int64_t)(ctx->split_chunk_bytes / per_h);
+    H = std::max<int64_t>(kWave, H / kWave * kWave);
+    H = std::min<int64_t>(H, (src.n_items + kWave - 1) / kWave * kWave);
+    void* scratch = nullptr;
+    BSMPS_TRY(bsmps_ctx_scratch(ctx, per_h * (size_t)H, &scratch));
+    double* fdata = reinterpret_cast<double*>(scratch);
+    double* vals = fdata + 2 * (size_t)fp->data_slots * H;
+    for (int64_t c0 = 0; c0 < src.n_items; c0 += H) {
+        const int64_t nh = std::min<int64_t>(H, src.n_items - c0);
+        SplitOut so{fdata, H, nh, fp->d_front, fp->abot_k, fp->abot_off};
+        const unsigned blocks = (unsigned)((nh + per - 1) / per);
+        switch (m) {
+#define BSMPS_SPLIT_CASE(MM)                                                                                  \
+    case MM:                                                                                                  \
+        if (lds > 64 * 1024)                                                                                  \
+            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S, true>,                        \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
+        hipLaunchKernelGGL((haf_group_kernel<MM, S, true>), dim3(blocks), dim3(kWave), lds, ctx->stream, src, c0, \
+                           fp->lay, fp->d_ops, G, so);                                                        \
+        BSMPS_HIP(hipGetLastError());                                                                         \
+        for (size_t t = 0; t < fp->types.size(); ++t)                                                         \
+            BSMPS_TRY(launch_subtrees_m<MM>(ctx->stream, fp->types[t].x, fp->types[t].y,                      \
+                                            dim3((unsigned)((nh + kWave - 1) / kWave), fp->type_count[t]),    \
+                                            fdata, H, nh, fp->d_front, fp->d_tlist + fp->type_first[t],       \
+                                            fp->abot_k, fp->abot_off, vals));                                 \
+        break;
+            BSMPS_SPLIT_CASE(6)
+            BSMPS_SPLIT_CASE(7)
+            BSMPS_SPLIT_CASE(8)
+            BSMPS_SPLIT_CASE(9)
+            BSMPS_SPLIT_CASE(10)
+#undef BSMPS_SPLIT_CASE
+            default:
+                bsmps_set_error("frontier-split hafnian path supports sizes 12..20, got %d", 2 * m);
+                return BSMPS_ERR_INVALID;
+        }
+        hipLaunchKernelGGL((haf_combine_kernel<S>), dim3((unsigned)((nh + kWave - 1) / kWave)), dim3(kWave), 0,
+                           ctx->stream, src, c0, nh, H, vals, fp->d_comb, fp->n_comb);
+        BSMPS_HIP(hipGetLastError());
+    }
+    return BSMPS_OK;
+}
+
 template <class S>
 int launch_hafnians(bsmps_ctx* ctx, const S& src, int n) {
     if (n % 2) return BSMPS_OK;  // odd sizes are zero; callers zero-fill
     const int m = n / 2;
     if (m <= ctx->lane_max_m || m <= 2) return launch_lane(ctx, src, m);
+    if (m >= kSplitMinM && m <= kSplitMaxM && ctx->split_by_m[m] && src.n_items > 0) {
+        FrontierPlan* fp = nullptr;
+        BSMPS_TRY(get_fplan(ctx->device, m, &fp));
+        if (fp->ok) return launch_split(ctx, src, m, fp);
+    }
     return launch_group(ctx, src, m);
 }
 

@@ -17,6 +17,7 @@
 #ifndef BSMPS_H
 #define BSMPS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -53,6 +54,14 @@ int bsmps_version(void);
  * for every size (bsmps_set_tuning) or for size n only (bsmps_set_group_lanes). */
 int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
 int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
+/* Hafnians of size n in 12..20 take the frontier-split path when `enable` is 1 (default): the
+ * cooperative lanes run the upper part of the recursion tree and hand each subtree small
+ * enough for one lane's registers to a one-lane-per-subtree kernel; results are identical to
+ * the unsplit path (no reference counterpart). */
+int bsmps_set_split(bsmps_ctx *ctx, int n, int enable);
+/* Device scratch the split path may hold (default 1 GiB, at least 1 MiB): batches of
+ * hafnians are processed in chunks whose frontier data fits it. */
+int bsmps_set_split_budget(bsmps_ctx *ctx, size_t bytes);
 
 /* ---- hafnians (L3) --------------------------------------------------------------------- */
 

@@ -37,5 +37,7 @@ def test_error_paths_without_device():
     assert b"set_tuning" in lib.bsmps_last_error()
     assert lib.bsmps_synchronize(None) == _lib.BSMPS_ERR_INVALID
     assert lib.bsmps_create(0, None, None) == _lib.BSMPS_ERR_INVALID
+    assert lib.bsmps_set_split(None, 12, 1) == _lib.BSMPS_ERR_INVALID
+    assert lib.bsmps_set_split_budget(None, 1 << 30) == _lib.BSMPS_ERR_INVALID
     with pytest.raises(_lib.BsmpsError):
         _lib.check(lib.bsmps_synchronize(None))

@@ -17,6 +17,9 @@ def ctx():
     c = runtime.context()
     yield c
     c.set_tuning(4, 64)
+    for n in range(12, 21, 2):
+        c.call("bsmps_set_split", n, 1)
+    c.call("bsmps_set_split_budget", 1 << 30)
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -74,6 +77,47 @@ def test_many_items_all_paths_agree(ctx):
             assert same_bits(o, outs[0])
 
 
+@pytest.mark.parametrize("n", [12, 14, 16, 18, 20])
+def test_split_matches_unsplit_and_oracle(ctx, n):
+    """The frontier-split path (upper tree on cooperating lanes, subtrees one per lane) gives
+    the same bits as the unsplit cooperative path and the C oracle, for every lane count and
+    when the batch is cut into many chunks."""
+    from bosonsupremacy_amd import _lib, mps
+    ctx.set_tuning(4, 64)
+    rng = np.random.default_rng(100 + n)
+    ld = 30
+    S = (rng.normal(size=(ld, ld)) + 1j * rng.normal(size=(ld, ld))) * 0.4
+    count = {12: 3000, 14: 1500, 16: 700, 18: 300, 20: 130}[n]
+    tg = rng.integers(0, ld, size=(count, n)).astype(np.int32)
+    den = rng.uniform(0.5, 2.0, size=count)
+    k = 12
+    want = c_oracle.hafnian_targets(S, tg[:k]) / den[:k]
+    outs = {}
+    for split in (0, 1):
+        ctx.call("bsmps_set_split", n, split)
+        for g in (16, 32, 64):
+            ctx.call("bsmps_set_group_lanes", n, g)
+            print(f"n={n} split={split} lanes={g}", flush=True)
+            try:
+                vals, _, _ = mps.A_elem(S, tg, den)
+            except _lib.BsmpsError as e:  # LDS of 64/g matrices per workgroup too large
+                assert e.status == _lib.BSMPS_ERR_UNSUPPORTED
+                continue
+            outs[(split, g)] = vals
+            assert same_bits(vals[:k], want), (split, g)
+    assert (1, 64) in outs and (0, 64) in outs
+    ctx.call("bsmps_set_split", n, 1)
+    ctx.call("bsmps_set_group_lanes", n, 64)
+    ctx.call("bsmps_set_split_budget", 1 << 20)
+    print(f"n={n} split, 1 MiB budget", flush=True)
+    chunked, _, _ = mps.A_elem(S, tg, den)
+    ctx.call("bsmps_set_split_budget", 1 << 30)
+    ref = outs[(0, 64)]
+    for key, vals in outs.items():
+        assert same_bits(vals, ref), key
+    assert same_bits(chunked, ref)
+
+
 def test_sigma_select_and_odd_sizes(ctx):
     from bosonsupremacy_amd import mps
     rng = np.random.default_rng(2)
@@ -92,3 +136,7 @@ def test_invalid_sizes_raise(ctx):
         mps.A_elem(np.eye(4), np.zeros((2, 26), np.int32), np.ones(2))
     with pytest.raises(_lib.BsmpsError):
         ctx.set_tuning(6, 64)
+    with pytest.raises(_lib.BsmpsError):
+        ctx.call("bsmps_set_split", 22, 1)
+    with pytest.raises(_lib.BsmpsError):
+        ctx.call("bsmps_set_split_budget", 1000)
