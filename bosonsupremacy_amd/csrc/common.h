@@ -63,10 +63,19 @@ struct bsmps_ctx {
     void* hws;
     size_t hws_bytes;
     size_t split_chunk_bytes;  // upper bound of `hws` per chunk of hafnians
+    // two pinned host staging buffers for small per-call tables, used alternately; an event
+    // marks when the copy out of each has completed
+    void* pin[2];
+    size_t pin_bytes[2];
+    hipEvent_t pin_ev[2];
+    int pin_next;
 };
 
 int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out);
 int bsmps_ctx_scratch(bsmps_ctx* ctx, size_t bytes, void** out);
+// Asynchronous host -> device copy of a small table through pinned staging: returns once the
+// table is staged, without waiting for the stream.
+int bsmps_ctx_upload(bsmps_ctx* ctx, const void* src, size_t bytes, void* dst);
 
 #define BSMPS_TRY(expr)                      \
     do {                                     \

@@ -2,6 +2,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "common.h"
 
@@ -51,6 +52,26 @@ int bsmps_ctx_scratch(bsmps_ctx* ctx, size_t bytes, void** out) {
     return BSMPS_OK;
 }
 
+int bsmps_ctx_upload(bsmps_ctx* ctx, const void* src, size_t bytes, void* dst) {
+    if (bytes == 0) return BSMPS_OK;
+    const int i = ctx->pin_next;
+    ctx->pin_next ^= 1;
+    if (ctx->pin_ev[i]) BSMPS_HIP(hipEventSynchronize(ctx->pin_ev[i]));  // the copy two calls ago
+    else BSMPS_HIP(hipEventCreateWithFlags(&ctx->pin_ev[i], hipEventDisableTiming));
+    if (bytes > ctx->pin_bytes[i]) {
+        if (ctx->pin[i]) BSMPS_HIP(hipHostFree(ctx->pin[i]));
+        ctx->pin[i] = nullptr;
+        ctx->pin_bytes[i] = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        BSMPS_HIP(hipHostMalloc(&ctx->pin[i], want, hipHostMallocDefault));
+        ctx->pin_bytes[i] = want;
+    }
+    memcpy(ctx->pin[i], src, bytes);
+    BSMPS_HIP(hipMemcpyAsync(dst, ctx->pin[i], bytes, hipMemcpyHostToDevice, ctx->stream));
+    BSMPS_HIP(hipEventRecord(ctx->pin_ev[i], ctx->stream));
+    return BSMPS_OK;
+}
+
 extern "C" const char* bsmps_last_error(void) { return g_last_error; }
 
 extern "C" int bsmps_version(void) { return 1; }
@@ -77,7 +98,13 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     for (int m = 0; m < 16; ++m) c->split_by_m[m] = (m >= 6 && m <= 10) ? 1 : 0;
     c->hws = nullptr;
     c->hws_bytes = 0;
-    c->split_chunk_bytes = (size_t)1 << 30;
+    c->split_chunk_bytes = (size_t)4 << 30;
+    for (int i = 0; i < 2; ++i) {
+        c->pin[i] = nullptr;
+        c->pin_bytes[i] = 0;
+        c->pin_ev[i] = nullptr;
+    }
+    c->pin_next = 0;
     c->max_lds = 64 * 1024;
     {
         int v = 0;
@@ -98,6 +125,10 @@ extern "C" int bsmps_destroy(bsmps_ctx* ctx) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = BSMPS_ERR_HIP;
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->hws) hipFree(ctx->hws);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->pin[i]) hipHostFree(ctx->pin[i]);
+        if (ctx->pin_ev[i]) hipEventDestroy(ctx->pin_ev[i]);
+    }
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return st;

@@ -34,6 +34,25 @@ constexpr int kWave = 64;
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int tri(int J, int K) { return J * (J - 1) / 2 + K; }
 
+// Highest polynomial coefficient of matrix entry (J, K), J > K, that can still reach the
+// leaf value g[m] of a 2m x 2m hafnian.  A coefficient of degree t reaches g only by raising
+// its degree: by 1 when the entry is a pivot pair (2p+1, 2p) feeding g at level p, by at
+// least 2 for any other entry (it first feeds some child entry as a pivot).  Coefficients
+// above this bound never contribute to the result, so they are not computed; every
+// computed value is produced by the same operations in the same order as before.
+__host__ __device__ constexpr int live_degree(int m, int J, int K) {
+    return (J == K + 1 && (K & 1) == 0) ? m - 1 : m - 2;
+}
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits for the
+// workgroup's outstanding global stores, which the cooperative kernel issues between its
+// phases (frontier data) and never reads back in the same launch.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------------------
 // Item sources: where an item's matrix comes from and where its result goes.
 // fetch() fills the target indices t[0..n-1] and returns the base/leading dimension of the
@@ -167,6 +186,7 @@ struct Node {
         if constexpr (L == M) return G >= M;
         else return Node<M, L + 1, D, G>::nonzero() || Node<M, L + 1, D2, G2>::nonzero();
     }
+    static constexpr int cap(int J, int K) { return live_degree(M, J, K); }
 
     template <class AF>
     __device__ __forceinline__ static cd coef(const AF& A, const cd (&P)[NE][NC], int J, int K, int u) {
@@ -193,6 +213,7 @@ struct Node {
                     for (int K = 2 * L + 2; K < J; ++K) {
 #pragma unroll
                         for (int t = 1; t <= D2; ++t) {
+                            if (t > cap(J, K)) continue;
                             cd acc = (t <= D) ? P[tri(J, K)][t - 1] : czero();
 #pragma unroll
                             for (int u = 0; u < t; ++u) {
@@ -311,7 +332,7 @@ Layout make_layout(int m) {
     for (int l = 1; l < m; ++l) {
         const int s = 2 * m - 2 * l;
         const int n_ent = s * (s - 1) / 2;
-        const int cap = std::min(m, (1 << std::min(l, 20)) - 1);
+        const int cap = std::min(m - 1, (1 << std::min(l, 20)) - 1);  // live coefficients only
         L.off_S[l] = off;
         L.cap[l] = cap;
         off += n_ent * cap;
@@ -376,6 +397,7 @@ int get_plan(int device, int m, GroupPlan** out) {
         sb.m = m;
         sb.gen(0, 0, 0, 0, 0, 1);
         sb.ops.push_back(make_int4(3, 0, 0, 0));
+        sb.ops.push_back(make_int4(3, 0, 0, 0));  // read ahead by the kernel
         GroupPlan p;
         p.m = m;
         p.lay = make_layout(m);
@@ -397,68 +419,60 @@ __device__ __forceinline__ void decode_pair(int i, int& J, int& K) {
     K = i - j * (j - 1) / 2;
 }
 
-// Right-branch child-matrix entry (J, K): all coefficients t = 1..D2 of
+// Right-branch child-matrix entry (J, K): its live coefficients t = 1..min(D2, live degree) of
 //   c(J,K)[t] = b(J,K)[t] + sum_u (b(J,b0)[u] b(K,b1)[v] + b(K,b0)[u] b(J,b1)[v]),  v = t-1-u,
 // accumulated in ascending u as solve() does (MPS_cpu.py:179-186).  The node's four pivot
 // polynomials are loaded once into registers; D (the node's degree) and D2 are compile-time,
 // which is possible because a node's degree is always 2^r - 1 or m.
 template <int D, int D2>
 __device__ __forceinline__ void child_entry(const cd* Ar, const cd* So, int obase, int ocap, cd* dst, int J,
-                                            int K, int b0, int b1) {
+                                            int K, int b0, int b1, int tmax) {
     auto co = [&](int r, int c, int u) -> cd {
         return u == 0 ? Ar[tri(r, c)] : So[tri(r - obase, c - obase) * ocap + (u - 1)];
     };
-    cd X[D + 1], Y[D + 1], Z[D + 1], W[D + 1];
+    constexpr int DU = D < D2 - 1 ? D : D2 - 1;  // highest pivot degree any t <= D2 uses
+    cd X[DU + 1], Y[DU + 1], Z[DU + 1], W[DU + 1];
 #pragma unroll
-    for (int u = 0; u <= D; ++u) {
-        X[u] = co(J, b0, u);
-        Y[u] = co(K, b1, u);
-        Z[u] = co(K, b0, u);
-        W[u] = co(J, b1, u);
+    for (int u = 0; u <= DU; ++u) {
+        const bool need = u < tmax;
+        X[u] = need ? co(J, b0, u) : czero();
+        Y[u] = need ? co(K, b1, u) : czero();
+        Z[u] = need ? co(K, b0, u) : czero();
+        W[u] = need ? co(J, b1, u) : czero();
     }
 #pragma unroll
     for (int t = 1; t <= D2; ++t) {
-        cd acc = (t <= D) ? co(J, K, t) : czero();
+        if (t <= tmax) {
+            cd acc = (t <= D) ? co(J, K, t) : czero();
 #pragma unroll
-        for (int u = 0; u <= D; ++u) {
-            const int v = t - 1 - u;
-            if (v >= 0 && v <= D) acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+            for (int u = 0; u <= DU; ++u) {
+                const int v = t - 1 - u;
+                if (v >= 0 && v <= D && u <= D) acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+            }
+            dst[t - 1] = acc;
         }
-        dst[t - 1] = acc;
     }
 }
 
 template <int M>
 __device__ __forceinline__ void child_entry_dispatch(int D, const cd* Ar, const cd* So, int obase, int ocap, cd* dst,
-                                                     int J, int K, int b0, int b1) {
+                                                     int J, int K, int b0, int b1, int tmax) {
     if (D == M) {
-        child_entry<M, M>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+        child_entry<M, M>(Ar, So, obase, ocap, dst, J, K, b0, b1, tmax);
     } else if (D == 0) {
-        child_entry<0, cmin(M, 1)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+        child_entry<0, cmin(M, 1)>(Ar, So, obase, ocap, dst, J, K, b0, b1, tmax);
     } else if (D == 1) {
-        if constexpr (M > 1) child_entry<1, cmin(M, 3)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+        if constexpr (M > 1) child_entry<1, cmin(M, 3)>(Ar, So, obase, ocap, dst, J, K, b0, b1, tmax);
     } else if (D == 3) {
-        if constexpr (M > 3) child_entry<3, cmin(M, 7)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+        if constexpr (M > 3) child_entry<3, cmin(M, 7)>(Ar, So, obase, ocap, dst, J, K, b0, b1, tmax);
     } else if (D == 7) {
-        if constexpr (M > 7) child_entry<7, cmin(M, 15)>(Ar, So, obase, ocap, dst, J, K, b0, b1);
+        if constexpr (M > 7) child_entry<7, cmin(M, 15)>(Ar, So, obase, ocap, dst, J, K, b0, b1, tmax);
     }
 }
 
-// Where the split path's upper phase writes the frontier subtrees' inputs: complex slot `i` of
-// hafnian `h` (0 <= h < n_h, relative to the chunk's first item) lives at fdata[i * stride + h],
-// hafnian index fastest, so the one-lane-per-subtree kernels read it coalesced.
-struct SplitOut {
-    double* fdata;
-    int64_t stride;
-    int64_t n_h;
-    const int4* front;  // per frontier node: {k | D << 8 | G << 16, owner | g_level << 8, offset, w > 0}
-    int abot_k;         // the bottom block holds A(J, K), J > K >= 2 abot_k, at slots abot_off..
-    int abot_off;
-};
-
-template <int M, class S, bool SPLIT>
+template <int M, class S>
 __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, Layout lay,
-                                                          const int4* __restrict__ ops, int G, SplitOut so) {
+                                                          const int4* __restrict__ ops, int G) {
     extern __shared__ cd smem[];
     const int PER = kWave / G;  // matrices per workgroup
     const int grp = threadIdx.x / G;
@@ -467,10 +481,9 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
     const int n = 2 * m;
     cd* R = smem + grp * lay.total;
     int* tg = reinterpret_cast<int*>(R + lay.off_tgt);
-    const int64_t hl = (int64_t)blockIdx.x * PER + grp;  // position within this launch
-    const int64_t item = item0 + hl;
-    const bool active = SPLIT ? hl < so.n_h : item < src.n_items;
-    const int64_t eff = active ? item : (SPLIT ? item0 + so.n_h - 1 : src.n_items - 1);
+    const int64_t item = item0 + (int64_t)blockIdx.x * PER + grp;
+    const bool active = item < src.n_items;
+    const int64_t eff = active ? item : src.n_items - 1;
 
     typename std::conditional<std::is_same<S, GammaSource>::value, GammaSource::Decoded, int>::type dc{};
     if constexpr (std::is_same<S, GammaSource>::value) dc = src.decode(eff);
@@ -480,7 +493,7 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
         else src.fetch(eff, k, tk);
         tg[k] = tk;
     }
-    __syncthreads();
+    lds_barrier();
     const double* Sg = src.base(eff);
     const int ld = src.ldim();
     for (int i = gl; i < m * (2 * m - 1); i += G) {
@@ -489,47 +502,13 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
         R[lay.off_A + i] = ld_cd(Sg, (int64_t)tg[J] * ld + tg[K]);
     }
     if (gl <= m) R[lay.off_g + gl] = gl == 0 ? cd{1.0, 0.0} : czero();  // g at level 0
-    __syncthreads();
-    if constexpr (SPLIT) {  // bottom block of the gathered matrix, read by every subtree
-        const int s0 = n - 2 * so.abot_k;
-        if (active)
-            for (int i = gl; i < s0 * (s0 - 1) / 2; i += G) {
-                int Jl, Kl;
-                decode_pair(i, Jl, Kl);
-                st_cd(so.fdata, (so.abot_off + i) * so.stride + hl,
-                      R[lay.off_A + tri(Jl + 2 * so.abot_k, Kl + 2 * so.abot_k)]);
-            }
-    }
-
+    lds_barrier();
     int sp = 0;  // value-stack depth (uniform)
+    int4 op = ops[0];
     for (int pc = 0;; ++pc) {
-        const int4 op = ops[pc];
+        const int4 next = ops[pc + 1];  // the schedule ends with two END ops; fetched ahead
         const int kind = op.x & 0xff;
         if (kind == 3) break;
-        if (SPLIT && kind == 4) {  // EMIT: copy a frontier node's polynomials and g out
-            const int4 fr = so.front[op.y];
-            const int k = fr.x & 0xff, D = (fr.x >> 8) & 0xff, Gd = (fr.x >> 16) & 0xff;
-            const int owner = fr.y & 0xff, glev = (fr.y >> 8) & 0xff;
-            const int s0 = n - 2 * k;
-            const int n_p = s0 * (s0 - 1) / 2 * D;
-            const cd* So = R + lay.off_S[owner > 0 ? owner : 1];
-            const int ocap = owner > 0 ? lay.cap[owner] : 0;
-            if (active)
-                for (int i = gl; i < n_p + Gd + 1; i += G) {
-                    cd v;
-                    if (i < n_p) {
-                        const int e = i / D, t = i - e * D;  // coefficient t + 1 of entry e
-                        int Jl, Kl;
-                        decode_pair(e, Jl, Kl);
-                        v = So[tri(Jl + 2 * (k - owner), Kl + 2 * (k - owner)) * ocap + t];
-                    } else {
-                        v = R[lay.off_g + glev * (m + 1) + (i - n_p)];
-                    }
-                    st_cd(so.fdata, (fr.z + i) * so.stride + hl, v);
-                }
-            __syncthreads();
-            continue;
-        }
         if (kind == 1) {  // LEAF: push w * g[m]
             if (gl == 0) {
                 const int gs = op.z & 0xff;
@@ -561,12 +540,15 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
             };
             const cd* gs = R + lay.off_g + gsrc * (m + 1);
             cd* gd = R + lay.off_g + gdst * (m + 1);
+            // One work item per child entry (all its live coefficients), then one per g update e[t].
             for (int it = gl; it < n_child + m; it += G) {
                 if (it < n_child) {
                     int Jl, Kl;
                     decode_pair(it, Jl, Kl);
                     const int J = Jl + b0 + 2, K = Kl + b0 + 2;
-                    child_entry_dispatch<M>(D, R + lay.off_A, So, obase, ocap, Sd + tri(Jl, Kl) * dcap, J, K, b0, b1);
+                    const int tmax = std::min(D2, live_degree(m, J, K));
+                    child_entry_dispatch<M>(D, R + lay.off_A, So, obase, ocap, Sd + tri(Jl, Kl) * dcap, J, K, b0, b1,
+                                            tmax);
                 } else {
                     const int t = it - n_child + 1;  // e[t], t = 1..m
                     cd acc = (t <= Gd) ? gs[t] : czero();
@@ -579,11 +561,12 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
             if (gl == 0) gd[0] = gs[0];
             (void)G2;
             (void)D2;
-            __syncthreads();
+            lds_barrier();
         }
+        op = next;
     }
-    __syncthreads();
-    if (!SPLIT && active && gl == 0) {
+    lds_barrier();
+    if (active && gl == 0) {
         const cd h = R[lay.off_val];
         if constexpr (std::is_same<S, GammaSource>::value) src.store_dec(dc, h);
         else src.store(item, h);
@@ -591,11 +574,142 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Frontier split, lower phase: one lane per (hafnian, frontier subtree).  A subtree rooted at
-// level K0 with node degree D (and g degree D: every frontier node the plan makes has G == D)
-// is evaluated by the same compile-time recursion as the lane path, from the polynomials and
-// g the upper phase wrote out; its value goes to vals[f * stride + h].
+// Frontier split (sizes 12..20).  The recursion tree is cut at "frontier" nodes whose subtree
+// fits one lane's registers.  The part above the frontier runs breadth-first, one launch per
+// tree level, with one lane per (hafnian, child-matrix entry): every node that owns new
+// polynomials (the root and each right child) keeps them in HBM, laid out
+// data[slot * stride + h] with the hafnian index h fastest, so all accesses are coalesced and
+// no lane idles.  Then one lane per (hafnian, frontier subtree) evaluates the subtree with the
+// compile-time recursion, and a last kernel adds the subtree values in tree order and applies
+// the item's epilogue.  Every value is produced by the same operations in the same order as
+// the depth-first recursion, so results are bit-identical to it.
 // ---------------------------------------------------------------------------------------
+
+// One RIGHT step of the upper tree: the node (level, degree D, g degree G) reads its matrix
+// from its owner's block (rows >= 2 plo, pcap coefficients per entry; po < 0: the root, whose
+// matrix is A alone) and its g from gsrc; the right child's block goes to pdst (dcap
+// coefficients per entry) and its g to gdst.
+struct BfsOp {
+    int level, D, D2, G;
+    int po, plo, pcap, gsrc;
+    int pdst, dcap, gdst, pad;
+};
+
+struct BfsData {
+    double* data;     // complex slots, slot-major
+    int64_t stride;   // hafnians per slot (chunk capacity)
+    int64_t n_h;      // hafnians in this chunk
+    int a_off;        // A(J, K) at slot a_off + tri(J, K)
+};
+
+__device__ __forceinline__ cd bfs_ld(const BfsData& bd, int slot, int64_t h) {
+    return ld_cd(bd.data, (int64_t)slot * bd.stride + h);
+}
+
+// Gather A and the root's g of every hafnian of the chunk (one lane per hafnian).
+template <int M, class S>
+__global__ __launch_bounds__(kWave) void haf_bfs_prep_kernel(S src, int64_t item0, BfsData bd, int rootg) {
+    const int64_t h = (int64_t)blockIdx.x * kWave + threadIdx.x;
+    if (h >= bd.n_h) return;
+    const int64_t item = item0 + h;
+    int t[2 * M];
+    if constexpr (std::is_same<S, GammaSource>::value) {
+        const auto dc = src.decode(item);
+#pragma unroll
+        for (int k = 0; k < 2 * M; ++k) src.fetch_dec(dc, k, t[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2 * M; ++k) src.fetch(item, k, t[k]);
+    }
+    const double* Sg = src.base(item);
+    const int ld = src.ldim();
+#pragma unroll
+    for (int J = 1; J < 2 * M; ++J)
+#pragma unroll
+        for (int K = 0; K < J; ++K)
+            st_cd(bd.data, (int64_t)(bd.a_off + tri(J, K)) * bd.stride + h, ld_cd(Sg, (int64_t)t[J] * ld + t[K]));
+#pragma unroll
+    for (int i = 0; i <= M; ++i)
+        st_cd(bd.data, (int64_t)(rootg + i) * bd.stride + h, i == 0 ? cd{1.0, 0.0} : czero());
+}
+
+// Child entry (J, K) of a RIGHT step (all its live coefficients), or the step's g update when
+// e < 0.  D and D2 are compile-time (a node's degree is 2^r - 1 or m).
+template <int M, int D, int D2>
+__device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int e, int64_t h) {
+    const int b0 = 2 * op.level, b1 = b0 + 1;
+    const int ob = 2 * op.plo;
+    auto co = [&](int r, int c, int u) -> cd {
+        return u == 0 ? bfs_ld(bd, bd.a_off + tri(r, c), h) : bfs_ld(bd, op.po + tri(r - ob, c - ob) * op.pcap + (u - 1), h);
+    };
+    if (e >= 0) {
+        int Jl, Kl;
+        decode_pair(e, Jl, Kl);
+        const int J = Jl + b0 + 2, K = Kl + b0 + 2;
+        const int tmax = cmin(D2, live_degree(M, J, K));
+        constexpr int DU = D < D2 - 1 ? D : D2 - 1;
+        cd X[DU + 1], Y[DU + 1], Z[DU + 1], W[DU + 1];
+#pragma unroll
+        for (int u = 0; u <= DU; ++u) {
+            const bool need = u < tmax;
+            X[u] = need ? co(J, b0, u) : czero();
+            Y[u] = need ? co(K, b1, u) : czero();
+            Z[u] = need ? co(K, b0, u) : czero();
+            W[u] = need ? co(J, b1, u) : czero();
+        }
+        const int64_t dst = (int64_t)(op.pdst + e * op.dcap) * bd.stride + h;
+#pragma unroll
+        for (int t = 1; t <= D2; ++t) {
+            if (t <= tmax) {
+                // c(J,K)[t] = b(J,K)[t] + sum_u (b(J,b0)[u] b(K,b1)[v] + b(K,b0)[u] b(J,b1)[v]),
+                // v = t-1-u, ascending u (MPS_cpu.py:179-186)
+                cd acc = (t <= D) ? co(J, K, t) : czero();
+#pragma unroll
+                for (int u = 0; u <= DU; ++u) {
+                    const int v = t - 1 - u;
+                    if (v >= 0 && v <= D && u <= D) acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+                }
+                st_cd(bd.data, dst + (int64_t)(t - 1) * bd.stride, acc);
+            }
+        }
+    } else {
+        // e = g + sum_{u,v} g[u] b(b1,b0)[v] into e[u+v+1] (MPS_cpu.py:175-178)
+        cd g[M + 1], p[D + 1];
+#pragma unroll
+        for (int i = 0; i <= M; ++i) g[i] = i <= op.G ? bfs_ld(bd, op.gsrc + i, h) : czero();
+#pragma unroll
+        for (int v = 0; v <= D; ++v) p[v] = v < M ? co(b1, b0, v) : czero();
+        st_cd(bd.data, (int64_t)op.gdst * bd.stride + h, g[0]);
+#pragma unroll
+        for (int t = 1; t <= M; ++t) {
+            cd acc = (t <= op.G) ? g[t] : czero();
+#pragma unroll
+            for (int u = 0; u < t; ++u) {
+                const int v = t - 1 - u;
+                if (u <= op.G && v <= D) acc = cadd(acc, cmul(g[u], p[v]));
+            }
+            st_cd(bd.data, (int64_t)(op.gdst + t) * bd.stride + h, acc);
+        }
+    }
+}
+
+// One tree level of the upper phase: blockIdx.x = item (op, entry) of this level,
+// blockIdx.y = block of 64 hafnians.
+template <int M>
+__global__ __launch_bounds__(kWave) void haf_bfs_level_kernel(BfsData bd, const BfsOp* __restrict__ ops,
+                                                              const int2* __restrict__ items) {
+    const int64_t h = (int64_t)blockIdx.y * kWave + threadIdx.x;
+    if (h >= bd.n_h) return;
+    const int2 it = items[blockIdx.x];
+    const BfsOp op = ops[it.x];
+    // the plan only makes degrees 0, 1, 3, 7 and M (checked on the host); M first, since it
+    // can coincide with 7
+    if (op.D == M) bfs_item<M, M, M>(bd, op, it.y, h);
+    else if (op.D == 0) bfs_item<M, 0, cmin(M, 1)>(bd, op, it.y, h);
+    else if (op.D == 1) bfs_item<M, 1, cmin(M, 3)>(bd, op, it.y, h);
+    else if (op.D == 3) { if constexpr (M > 3) bfs_item<M, 3, cmin(M, 7)>(bd, op, it.y, h); }
+    else if (op.D == 7) { if constexpr (M > 7) bfs_item<M, 7, cmin(M, 15)>(bd, op, it.y, h); }
+}
 
 template <int OFF>
 struct LdsAOff {  // A(J, K) of a subtree whose rows start at OFF, [entry][lane] in LDS
@@ -603,11 +717,10 @@ struct LdsAOff {  // A(J, K) of a subtree whose rows start at OFF, [entry][lane]
     __device__ __forceinline__ cd operator()(int J, int K) const { return base[tri(J - OFF, K - OFF) * kWave]; }
 };
 
+// Frontier node: {k | D << 8 | G << 16 | (w > 0) << 24, po, plo | pcap << 8, g slot}.
 template <int M, int K0, int D>
-__global__ __launch_bounds__(kWave) void haf_subtree_kernel(const double* __restrict__ fdata, int64_t stride,
-                                                            int64_t n_h, const int4* __restrict__ front,
-                                                            const int* __restrict__ tlist, int abot_k,
-                                                            int abot_off, double* __restrict__ vals) {
+__global__ __launch_bounds__(kWave) void haf_subtree_kernel(BfsData bd, const int4* __restrict__ front,
+                                                            const int* __restrict__ tlist, double* __restrict__ vals) {
     constexpr int S0 = 2 * (M - K0);
     constexpr int NA = S0 * (S0 - 1) / 2;
     __shared__ cd lds[NA * kWave];
@@ -615,28 +728,26 @@ __global__ __launch_bounds__(kWave) void haf_subtree_kernel(const double* __rest
     const int64_t h = (int64_t)blockIdx.x * kWave + lane;
     const int f = tlist[blockIdx.y];
     const int4 fr = front[f];
-    if (h >= n_h) return;
-    const int ak = 2 * abot_k;
+    if (h >= bd.n_h) return;
 #pragma unroll
     for (int J = 2 * K0 + 1; J < 2 * M; ++J)
 #pragma unroll
-        for (int K = 2 * K0; K < J; ++K)
-            lds[tri(J - 2 * K0, K - 2 * K0) * kWave + lane] = ld_cd(fdata, (abot_off + tri(J - ak, K - ak)) * stride + h);
+        for (int K = 2 * K0; K < J; ++K) lds[tri(J - 2 * K0, K - 2 * K0) * kWave + lane] = bfs_ld(bd, bd.a_off + tri(J, K), h);
     LdsAOff<2 * K0> A{lds + lane};
     cd P[LaneDims<M>::NE][LaneDims<M>::NC];
-    const int64_t base = fr.z;
+    const int po = fr.y, ob = 2 * (fr.z & 0xff), pcap = fr.z >> 8;
 #pragma unroll
     for (int J = 2 * K0 + 1; J < 2 * M; ++J)
 #pragma unroll
         for (int K = 2 * K0; K < J; ++K)
 #pragma unroll
             for (int t = 0; t < D; ++t)
-                P[tri(J, K)][t] = ld_cd(fdata, (base + tri(J - 2 * K0, K - 2 * K0) * D + t) * stride + h);
+                if (t < live_degree(M, J, K)) P[tri(J, K)][t] = bfs_ld(bd, po + tri(J - ob, K - ob) * pcap + t, h);
     cd g[M + 1];
 #pragma unroll
-    for (int t = 0; t <= M; ++t) g[t] = t <= D ? ld_cd(fdata, (base + NA * D + t) * stride + h) : czero();
-    const double w = fr.w ? 1.0 : -1.0;
-    st_cd(vals, (int64_t)f * stride + h, Node<M, K0, D, D>::run(A, P, g, w));
+    for (int t = 0; t <= M; ++t) g[t] = t <= D ? bfs_ld(bd, fr.w + t, h) : czero();
+    const double w = (fr.x >> 24) & 1 ? 1.0 : -1.0;
+    st_cd(vals, (int64_t)f * bd.stride + h, Node<M, K0, D, D>::run(A, P, g, w));
 }
 
 // Frontier split, last phase: add the subtree values in the order of the recursion tree
@@ -715,12 +826,12 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
 #define BSMPS_GROUP_CASE(MM)                                                                                  \
     case MM: {                                                                                                \
         if (lds > 64 * 1024)                                                                                  \
-            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S, false>,                       \
+            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S>,                       \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
         for (int64_t b0 = 0; b0 < blocks; b0 += kMaxGrid) {                                                   \
             const int64_t nb = std::min<int64_t>(kMaxGrid, blocks - b0);                                      \
-            hipLaunchKernelGGL((haf_group_kernel<MM, S, false>), dim3((unsigned)nb), dim3(kWave), lds, ctx->stream, \
-                               src, b0 * per, plan->lay, plan->d_ops, G, SplitOut{});                         \
+            hipLaunchKernelGGL((haf_group_kernel<MM, S>), dim3((unsigned)nb), dim3(kWave), lds, ctx->stream, \
+                               src, b0 * per, plan->lay, plan->d_ops, G);                                     \
             BSMPS_HIP(hipGetLastError());                                                                     \
         }                                                                                                     \
         break;                                                                                                \
@@ -743,37 +854,37 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
     return BSMPS_OK;
 }
 
-// Frontier-split plan of one size: the upper schedule (RIGHT / EMIT / END ops), the frontier
-// nodes with their data offsets, the combine program and the frontier nodes grouped by type.
-// A node is a frontier node when its subtree fits one lane: depth <= 4 with degree <= 1,
-// depth <= 3 with degree <= 3, or depth <= 2.
-struct FrontierBuilder {
+// Frontier-split plan of one size: storage slots of every owner node, the RIGHT steps grouped
+// by tree level with their work items, the frontier nodes grouped by subtree shape, and the
+// combine program.  A node is a frontier node when its subtree fits one lane: depth <= 4 with
+// degree <= 1, depth <= 3 with degree <= 3, or depth <= 2.
+struct BfsBuilder {
     int m;
     ScheduleBuilder nzb;
-    std::vector<int4> ops;
-    std::vector<int4> front;  // {k | D << 8 | G << 16, owner | g_level << 8, offset, w > 0}
-    std::vector<int> comb;
     int slots = 0;
-    bool is_frontier(int level, int D) const {
+    std::vector<BfsOp> ops;   // depth-first order
+    std::vector<int4> front;  // {k | D << 8 | G << 16 | (w > 0) << 24, po, plo | pcap << 8, g slot}
+    std::vector<int> comb;
+    static bool is_frontier(int m, int level, int D) {
         const int depth = m - level;
         return (depth <= 4 && D <= 1) || (depth <= 3 && D <= 3) || depth <= 2;
     }
-    void gen(int level, int owner, int D, int G, int gl, int w) {
-        if (is_frontier(level, D)) {
-            const int s0 = 2 * m - 2 * level;
+    void gen(int level, int po, int plo, int pcap, int gslot, int D, int G, int w) {
+        if (is_frontier(m, level, D)) {
             comb.push_back((int)front.size());
-            ops.push_back(make_int4(4, (int)front.size(), 0, 0));
-            front.push_back(make_int4(level | (D << 8) | (G << 16), owner | (gl << 8), slots, w > 0));
-            slots += s0 * (s0 - 1) / 2 * D + G + 1;
+            front.push_back(make_int4(level | (D << 8) | (G << 16) | ((w > 0) << 24), po, plo | (pcap << 8), gslot));
             return;
         }
         const int D2 = std::min(m, 2 * D + 1), G2 = std::min(m, G + D + 1);
         const bool lnz = nzb.nonzero(level + 1, D, G), rnz = nzb.nonzero(level + 1, D2, G2);
-        if (lnz) gen(level + 1, owner, D, G, gl, -w);
+        if (lnz) gen(level + 1, po, plo, pcap, gslot, D, G, -w);
         if (rnz) {
-            ops.push_back(make_int4(0 | (level << 8) | (owner << 16), D | (D2 << 8) | (G << 16) | (G2 << 24),
-                                    gl | ((level + 1) << 8) | ((w > 0) << 16), 0));
-            gen(level + 1, level + 1, D2, G2, level + 1, w);
+            const int s0 = 2 * m - 2 * (level + 1);
+            const int dcap = std::min(m - 1, D2);
+            BfsOp op{level, D, D2, G, po, plo, pcap, gslot, slots, dcap, slots + s0 * (s0 - 1) / 2 * dcap, 0};
+            slots += s0 * (s0 - 1) / 2 * dcap + m + 1;
+            ops.push_back(op);
+            gen(level + 1, op.pdst, level + 1, dcap, op.gdst, D2, G2, w);
         }
         if (lnz && rnz) comb.push_back(-1);
     }
@@ -790,17 +901,18 @@ bool subtree_compiled(int M, int K0, int D) {
 
 struct FrontierPlan {
     bool ok;
-    Layout lay;
-    int4* d_ops;
+    int slots;                       // complex slots per hafnian (A, root g, owner blocks)
+    int a_off, rootg;
+    BfsOp* d_ops;
+    int2* d_items;                   // (op, entry or -1), grouped by level
+    std::vector<int> level_first, level_count;
     int4* d_front;
+    int n_front;
+    int* d_tlist;
+    std::vector<int2> types;         // (K0, D)
+    std::vector<int> type_first, type_count;
     int* d_comb;
     int n_comb;
-    int* d_tlist;
-    std::vector<int2> types;  // (K0, D)
-    std::vector<int> type_first, type_count;
-    int n_front;
-    int data_slots;  // frontier inputs + bottom block, per hafnian
-    int abot_k, abot_off;
 };
 
 std::map<std::pair<int, int>, FrontierPlan> g_fplans;  // (device, m)
@@ -810,27 +922,41 @@ int get_fplan(int device, int m, FrontierPlan** out) {
     auto key = std::make_pair(device, m);
     auto it = g_fplans.find(key);
     if (it == g_fplans.end()) {
-        FrontierBuilder fb;
+        BfsBuilder fb;
         fb.m = m;
         fb.nzb.m = m;
-        fb.gen(0, 0, 0, 0, 0, 1);
-        fb.ops.push_back(make_int4(3, 0, 0, 0));
         FrontierPlan p{};
         p.ok = true;
-        p.lay = make_layout(m);
+        p.a_off = 0;
+        p.rootg = m * (2 * m - 1);
+        fb.slots = p.rootg + m + 1;
+        fb.gen(0, -1, 0, 0, p.rootg, 0, 0, 1);
+        p.slots = fb.slots;
+        // work items of each level: every child entry of every RIGHT step, then its g update
+        std::vector<int2> items;
+        for (int l = 0; l < m; ++l) {
+            const int first = (int)items.size();
+            for (size_t o = 0; o < fb.ops.size(); ++o) {
+                if (fb.ops[o].level != l) continue;
+                const int s0 = 2 * m - 2 * (l + 1);
+                for (int e = 0; e < s0 * (s0 - 1) / 2; ++e) items.push_back(make_int2((int)o, e));
+                items.push_back(make_int2((int)o, -1));
+            }
+            if ((int)items.size() > first) {
+                p.level_first.push_back(first);
+                p.level_count.push_back((int)items.size() - first);
+            }
+        }
+        for (const BfsOp& op : fb.ops)
+            if (!(op.D == 0 || op.D == 1 || op.D == 3 || op.D == 7 || op.D == m)) p.ok = false;
         p.n_front = (int)fb.front.size();
-        p.abot_k = m;
         std::map<std::pair<int, int>, std::vector<int>> by_type;
         for (int f = 0; f < p.n_front; ++f) {
             const int4 fr = fb.front[f];
             const int k = fr.x & 0xff, D = (fr.x >> 8) & 0xff, G = (fr.x >> 16) & 0xff;
             if (G != D || !subtree_compiled(m, k, D)) p.ok = false;
-            p.abot_k = std::min(p.abot_k, k);
             by_type[std::make_pair(k, D)].push_back(f);
         }
-        const int sb = 2 * m - 2 * p.abot_k;
-        p.abot_off = fb.slots;
-        p.data_slots = fb.slots + sb * (sb - 1) / 2;
         std::vector<int> tlist;
         for (auto& kv : by_type) {
             p.types.push_back(make_int2(kv.first.first, kv.first.second));
@@ -845,14 +971,17 @@ int get_fplan(int device, int m, FrontierPlan** out) {
             max_depth = std::max(max_depth, depth);
         }
         if (max_depth > kCombineStack || depth != 1) p.ok = false;
-        BSMPS_HIP(hipMalloc(&p.d_ops, sizeof(int4) * fb.ops.size()));
-        BSMPS_HIP(hipMemcpy(p.d_ops, fb.ops.data(), sizeof(int4) * fb.ops.size(), hipMemcpyHostToDevice));
-        BSMPS_HIP(hipMalloc(&p.d_front, sizeof(int4) * fb.front.size()));
-        BSMPS_HIP(hipMemcpy(p.d_front, fb.front.data(), sizeof(int4) * fb.front.size(), hipMemcpyHostToDevice));
-        BSMPS_HIP(hipMalloc(&p.d_comb, sizeof(int) * fb.comb.size()));
-        BSMPS_HIP(hipMemcpy(p.d_comb, fb.comb.data(), sizeof(int) * fb.comb.size(), hipMemcpyHostToDevice));
-        BSMPS_HIP(hipMalloc(&p.d_tlist, sizeof(int) * tlist.size()));
-        BSMPS_HIP(hipMemcpy(p.d_tlist, tlist.data(), sizeof(int) * tlist.size(), hipMemcpyHostToDevice));
+        auto upload = [](auto** dst, const auto& v) -> int {
+            const size_t bytes = sizeof(v[0]) * std::max<size_t>(v.size(), 1);
+            BSMPS_HIP(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+            if (!v.empty()) BSMPS_HIP(hipMemcpy(*dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice));
+            return BSMPS_OK;
+        };
+        BSMPS_TRY(upload(&p.d_ops, fb.ops));
+        BSMPS_TRY(upload(&p.d_items, items));
+        BSMPS_TRY(upload(&p.d_front, fb.front));
+        BSMPS_TRY(upload(&p.d_tlist, tlist));
+        BSMPS_TRY(upload(&p.d_comb, fb.comb));
         it = g_fplans.emplace(key, p).first;
     }
     *out = &it->second;
@@ -860,11 +989,10 @@ int get_fplan(int device, int m, FrontierPlan** out) {
 }
 
 template <int M>
-int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const double* fdata, int64_t stride, int64_t n_h,
-                      const int4* front, const int* tlist, int abot_k, int abot_off, double* vals) {
-#define BSMPS_SUBTREE(KK, DD)                                                                                 \
-    hipLaunchKernelGGL((haf_subtree_kernel<M, KK, DD>), grid, dim3(kWave), 0, st, fdata, stride, n_h, front,  \
-                       tlist, abot_k, abot_off, vals)
+int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const BfsData& bd, const int4* front,
+                      const int* tlist, double* vals) {
+#define BSMPS_SUBTREE(KK, DD) \
+    hipLaunchKernelGGL((haf_subtree_kernel<M, KK, DD>), grid, dim3(kWave), 0, st, bd, front, tlist, vals)
     if (K0 == M - 4 && D == 0) BSMPS_SUBTREE(M - 4, 0);
     else if (K0 == M - 4 && D == 1) BSMPS_SUBTREE(M - 4, 1);
     else if (K0 == M - 3 && D == 3) BSMPS_SUBTREE(M - 3, 3);
@@ -879,44 +1007,37 @@ int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const double* fd
     return BSMPS_OK;
 }
 
-// Frontier-split launch: per chunk of hafnians, the upper phase (cooperative lanes), one launch
-// per subtree shape, then the combine; the chunk is sized so its scratch fits the budget.
+// Frontier-split launch: per chunk of hafnians (sized so its HBM storage fits the budget),
+// the gather, one launch per upper-tree level, one per subtree shape, then the combine.
 template <class S>
 int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
-    const int G = ctx->group_lanes_by_m[m];
-    const int per = kWave / G;
-    const size_t lds = sizeof(cd) * (size_t)fp->lay.total * per;
-    if (lds > (size_t)ctx->max_lds) {
-        bsmps_set_error("hafnian of size %d needs %zu bytes of LDS per workgroup with %d lanes per matrix "
-                        "(device limit %d)", 2 * m, lds, G, ctx->max_lds);
-        return BSMPS_ERR_UNSUPPORTED;
-    }
-    const size_t per_h = sizeof(cd) * (size_t)(fp->data_slots + fp->n_front);
+    const size_t per_h = sizeof(cd) * (size_t)(fp->slots + fp->n_front);
     int64_t H = (int64_t)(ctx->split_chunk_bytes / per_h);
     H = std::max<int64_t>(kWave, H / kWave * kWave);
     H = std::min<int64_t>(H, (src.n_items + kWave - 1) / kWave * kWave);
     void* scratch = nullptr;
     BSMPS_TRY(bsmps_ctx_scratch(ctx, per_h * (size_t)H, &scratch));
-    double* fdata = reinterpret_cast<double*>(scratch);
-    double* vals = fdata + 2 * (size_t)fp->data_slots * H;
+    double* data = reinterpret_cast<double*>(scratch);
+    double* vals = data + 2 * (size_t)fp->slots * H;
     for (int64_t c0 = 0; c0 < src.n_items; c0 += H) {
         const int64_t nh = std::min<int64_t>(H, src.n_items - c0);
-        SplitOut so{fdata, H, nh, fp->d_front, fp->abot_k, fp->abot_off};
-        const unsigned blocks = (unsigned)((nh + per - 1) / per);
+        const unsigned hblocks = (unsigned)((nh + kWave - 1) / kWave);
+        BfsData bd{data, H, nh, fp->a_off};
         switch (m) {
 #define BSMPS_SPLIT_CASE(MM)                                                                                  \
     case MM:                                                                                                  \
-        if (lds > 64 * 1024)                                                                                  \
-            BSMPS_HIP(hipFuncSetAttribute((const void*)&haf_group_kernel<MM, S, true>,                        \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
-        hipLaunchKernelGGL((haf_group_kernel<MM, S, true>), dim3(blocks), dim3(kWave), lds, ctx->stream, src, c0, \
-                           fp->lay, fp->d_ops, G, so);                                                        \
+        hipLaunchKernelGGL((haf_bfs_prep_kernel<MM, S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, bd, \
+                           fp->rootg);                                                                        \
         BSMPS_HIP(hipGetLastError());                                                                         \
+        for (size_t l = 0; l < fp->level_first.size(); ++l) {                                                 \
+            hipLaunchKernelGGL((haf_bfs_level_kernel<MM>), dim3((unsigned)fp->level_count[l], hblocks), dim3(kWave), \
+                               0, ctx->stream, bd, fp->d_ops, fp->d_items + fp->level_first[l]);              \
+            BSMPS_HIP(hipGetLastError());                                                                     \
+        }                                                                                                     \
         for (size_t t = 0; t < fp->types.size(); ++t)                                                         \
             BSMPS_TRY(launch_subtrees_m<MM>(ctx->stream, fp->types[t].x, fp->types[t].y,                      \
-                                            dim3((unsigned)((nh + kWave - 1) / kWave), fp->type_count[t]),    \
-                                            fdata, H, nh, fp->d_front, fp->d_tlist + fp->type_first[t],       \
-                                            fp->abot_k, fp->abot_off, vals));                                 \
+                                            dim3(hblocks, fp->type_count[t]), bd, fp->d_front,                \
+                                            fp->d_tlist + fp->type_first[t], vals));                          \
         break;
             BSMPS_SPLIT_CASE(6)
             BSMPS_SPLIT_CASE(7)
@@ -928,8 +1049,8 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
                 bsmps_set_error("frontier-split hafnian path supports sizes 12..20, got %d", 2 * m);
                 return BSMPS_ERR_INVALID;
         }
-        hipLaunchKernelGGL((haf_combine_kernel<S>), dim3((unsigned)((nh + kWave - 1) / kWave)), dim3(kWave), 0,
-                           ctx->stream, src, c0, nh, H, vals, fp->d_comb, fp->n_comb);
+        hipLaunchKernelGGL((haf_combine_kernel<S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, nh, H, vals,
+                           fp->d_comb, fp->n_comb);
         BSMPS_HIP(hipGetLastError());
     }
     return BSMPS_OK;
@@ -1108,8 +1229,7 @@ extern "C" int bsmps_gamma_site(bsmps_ctx* ctx, const bsmps_gamma_site_args* a, 
     memcpy(p, roff.data(), sizeof(int32_t) * roff.size());
     p += sizeof(int32_t) * roff.size();
     memcpy(p, rcnt.data(), sizeof(int32_t) * rcnt.size());
-    BSMPS_HIP(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
-    BSMPS_HIP(hipStreamSynchronize(ctx->stream));  // `host` goes out of scope
+    BSMPS_TRY(bsmps_ctx_upload(ctx, host.data(), host.size(), ws));
     const int4* d_rects = reinterpret_cast<const int4*>(ws);
     const int32_t* d_loff = reinterpret_cast<const int32_t*>(reinterpret_cast<char*>(ws) + bytes_rects);
     const int32_t* d_roff = d_loff + loff.size();

@@ -11,8 +11,32 @@ the same LAPACK calls, on the same operands, in the same order as the reference:
 williamson = kron_cpu.py:46-89, blochmessiah = MPS_cpu.py:189-229,
 get_U2_sq_U1 = MPS_cpu.py:357-372, get_Sigma = MPS_cpu.py:305-312.
 """
+import functools
+
 import numpy as np
 from scipy.linalg import schur, sqrtm, svd
+
+try:
+    from threadpoolctl import ThreadpoolController
+    _BLAS = ThreadpoolController()
+except Exception:  # pragma: no cover - threadpoolctl ships with this environment
+    _BLAS = None
+
+
+def single_thread_blas(fn):
+    """Run ``fn`` with single-threaded BLAS/LAPACK.
+
+    OpenBLAS results for these matrix sizes (up to a few hundred) depend on how many threads
+    it splits a call over, so the same call gives different last bits on hosts with
+    different core counts.  Pinning the decompositions to one thread makes them
+    reproducible on any host, in any process (the host worker pool included)."""
+    @functools.wraps(fn)
+    def wrapper(*args, **kwargs):
+        if _BLAS is None:
+            return fn(*args, **kwargs)
+        with _BLAS.limit(limits=1, user_api="blas"):
+            return fn(*args, **kwargs)
+    return wrapper
 
 
 def sympmat(n):
@@ -39,6 +63,7 @@ def xpxp_to_xxpp(S):
     return S[ind]
 
 
+@single_thread_blas
 def williamson(V, tol=1e-11):
     """Williamson normal form of a real symmetric positive-definite V (xxpp).
 
@@ -77,6 +102,22 @@ def thermal_photons(nth, cutoff=20):
     return 1 / (nth + 1) * (nth / (nth + 1)) ** np.arange(cutoff)
 
 
+@single_thread_blas
+def cut_inputs(sq_cov, L, cutoff):
+    """Williamson of modes L..M-1 of sq_cov and their thermal photon distributions
+    [M-L, cutoff] (kron_cpu.py:96-106)."""
+    M = len(sq_cov) // 2
+    mode = np.arange(L, M)
+    modes = np.append(mode, mode + M)
+    D, S = williamson(sq_cov[np.ix_(modes, modes)])
+    nth = (np.diag(D) - 1) / 2
+    nth[nth < 0] = 0
+    n = M - L
+    tp = np.stack([np.array(thermal_photons(nth[i], cutoff)) for i in range(n)])
+    return tp, S
+
+
+@single_thread_blas
 def blochmessiah(S):
     """Bloch-Messiah decomposition S = uff @ dff @ vff of a real symplectic S (xxpp)."""
     N = S.shape[0]
@@ -105,6 +146,7 @@ def blochmessiah(S):
     return uff, dff, vff
 
 
+@single_thread_blas
 def get_U2_sq_U1(S_l, S_r):
     """Site unitary decomposition: S = (1 (+) S_l^-1) S_r = U2 . sq . U1 (complex form)."""
     M = len(S_r) // 2
@@ -119,6 +161,7 @@ def get_U2_sq_U1(S_l, S_r):
     return U2, sq, U1
 
 
+@single_thread_blas
 def get_Sigma(U2, sq, U1):
     """Fock-amplitude kernel of the site's Gaussian unitary, complex128 [2M', 2M']."""
     M = len(sq)
@@ -131,6 +174,7 @@ def get_Sigma(U2, sq, U1):
     return Sigma
 
 
+@single_thread_blas
 def site_kernel(k, M, sq_cov, S_cuts, S_full=None):
     """(Sigma, Z) of site k (MPS_cpu.py:395, 409-414, 439-442, 481-483).
 

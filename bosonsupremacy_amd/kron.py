@@ -22,7 +22,7 @@ import os
 import numpy as np
 import torch
 
-from . import gaussian, runtime
+from . import gaussian, hostpool, runtime
 
 ERR_TOL = 1e-12    # kron_cpu.py:95
 MAX_DIM = 10 ** 5  # kron_cpu.py:145
@@ -31,15 +31,7 @@ MAX_DIM = 10 ** 5  # kron_cpu.py:145
 def cut_spectrum_inputs(sq_cov, L, cutoff):
     """Williamson of modes L..M-1 and their thermal distributions [n, cutoff]
     (kron_cpu.py:96-106)."""
-    M = len(sq_cov) // 2
-    mode = np.arange(L, M)
-    modes = np.append(mode, mode + M)
-    D, S = gaussian.williamson(sq_cov[np.ix_(modes, modes)])
-    nth = (np.diag(D) - 1) / 2
-    nth[nth < 0] = 0
-    n = M - L
-    tp = np.stack([np.array(gaussian.thermal_photons(nth[i], cutoff)) for i in range(n)])
-    return tp, S
+    return gaussian.cut_inputs(sq_cov, L, cutoff)
 
 
 def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None):
@@ -87,15 +79,12 @@ def get_cumsum_kron(sq_cov, L, chi=100, max_dim=MAX_DIM, cutoff=6, err_tol=ERR_T
 
 
 def kron_all(sq_cov, d, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, timer=None):
-    """All cuts k = 0..M-2 (cut k keeps modes k+1..M-1), as kron_cpu.py:147-153."""
+    """All cuts k = 0..M-2 (cut k keeps modes k+1..M-1), as kron_cpu.py:147-153.  The
+    per-cut Williamson decompositions run in the host worker pool."""
     M = len(sq_cov) // 2
-    tps, Ss = [], []
-    for k in range(M - 1):
-        tp, S = cut_spectrum_inputs(sq_cov, k + 1, d)
-        tps.append(tp)
-        Ss.append(S)
-    spec = kron_cuts(tps, chi, max_dim, err_tol, timer=timer)
-    return [r for r, _ in spec], [n for _, n in spec], Ss
+    cuts = hostpool.cut_inputs_all(sq_cov, d)
+    spec = kron_cuts([tp for tp, _ in cuts], chi, max_dim, err_tol, timer=timer)
+    return [r for r, _ in spec], [n for _, n in spec], [S for _, S in cuts]
 
 
 def main(argv=None):

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 from scipy.special import factorial
 
-from . import _lib, gaussian, runtime
+from . import _lib, gaussian, hostpool, runtime
 
 ERR_TOL = 1e-10  # MPS_cpu.py:390
 
@@ -126,7 +126,9 @@ class _Labels:
         if offset:
             tg = np.where(tg != 0, tg + offset, 0).astype(np.int32)
         self.targets = tg
-        self.den = np.sqrt(np.prod(np.array(factorial(num)), axis=1)).astype(np.float64)
+        # factorial of each occupation through a table of the same scipy values
+        table = np.array(factorial(np.arange(int(num.max(initial=0)) + 1)), dtype=np.float64)
+        self.den = np.sqrt(np.prod(table[num], axis=1)).astype(np.float64)
         self.perm = np.argsort(self.count, kind='stable').astype(np.int32)
         self.class_count = np.bincount(self.count).astype(np.int32) if len(self.count) else np.zeros(1, np.int32)
 
@@ -145,16 +147,18 @@ def _dev_i32(a):
     a = np.ascontiguousarray(a, dtype=np.int32)
     if a.size == 0:
         a = np.zeros(1, np.int32)
-    return runtime.to_device(a)
+    return runtime.upload(a)
 
 
-def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None):
+def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None,
+                sigma_z=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
 
-    Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site."""
+    Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site.
+    ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel)."""
     ctx = runtime.context()
     dev = ctx.device
-    Sigma, Z = gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
+    Sigma, Z = sigma_z if sigma_z is not None else gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
     Mp = Sigma.shape[0] // 2
     Lambda = np.zeros(chi)
     if k == M - 1:
@@ -175,11 +179,12 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
         right = _Labels(num_pre.reshape(num_pre.shape[0], -1), Mp - 1)
     sqrt_fact = np.array([np.sqrt(factorial(j)) for j in np.arange(d)], dtype=np.float64)
 
-    sig_d = runtime.to_device(Sigma)
-    sf_d = runtime.to_device(sqrt_fact)
+    # uploads are asynchronous, so this host work overlaps the previous site's kernels
+    sig_d = runtime.upload(Sigma)
+    sf_d = runtime.upload(sqrt_fact)
     lt_d, lp_d = _dev_i32(left.targets), _dev_i32(left.perm)
     rt_d, rp_d = _dev_i32(right.targets), _dev_i32(right.perm)
-    lden_d, rden_d, lam_d = runtime.to_device(left.den), runtime.to_device(right.den), runtime.to_device(lam)
+    lden_d, rden_d, lam_d = runtime.upload(left.den), runtime.upload(right.den), runtime.upload(lam)
     lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
     rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
     args = _lib.GammaSiteArgs(
@@ -199,16 +204,28 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
 
 def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
-    outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them."""
+    outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them; when it is
+    None the spectra are computed too.
+
+    Pipelined: the host FP64 work of every site (Bloch-Messiah, Sigma) is queued on the
+    host worker pool as soon as the cut symplectics are known -- before the Schmidt spectra
+    run on the GPU -- and each site's hafnian launch is enqueued, asynchronously, as soon as
+    its Sigma and label tables are ready, so host and GPU work overlap."""
     from . import kron as kron_mod
     M = len(sq_cov) // 2
+    S_full_f = hostpool.williamson_S(sq_cov)
     if kron is None:
-        kron = kron_mod.kron_all(sq_cov, d, chi, max_dim, timer=timer)
+        cuts = hostpool.cut_inputs_all(sq_cov, d)
+        kS = [S for _, S in cuts]
+        site_f = hostpool.submit_sites(sq_cov, kS, S_full_f.result())
+        spec = kron_mod.kron_cuts([tp for tp, _ in cuts], chi, max_dim, timer=timer)
+        kron = ([r for r, _ in spec], [n for _, n in spec], kS)
+    else:
+        site_f = hostpool.submit_sites(sq_cov, kron[2], S_full_f.result())
     kres, knum, kS = kron
-    S_full = gaussian.williamson(sq_cov)[1]
     Gammas, Lambdas = [], []
     for k in range(M):
-        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full, timer=timer)
+        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result())
         Gammas.append(G)
         Lambdas.append(L)
     return Gammas, Lambdas, kron
@@ -228,10 +245,10 @@ def main(argv=None):
     kres = [np.load(path + f'res_{k}.npy').astype(np.float64) for k in range(M - 1)]
     knum = [np.load(path + f'num_{k}.npy') for k in range(M - 1)]
     kS = [np.load(path + f'S_{k}.npy') for k in range(M - 1)]
-    S_full = gaussian.williamson(sq_cov)[1]
+    site_f = hostpool.submit_sites(sq_cov, kS, hostpool.williamson_S(sq_cov).result())
     for k in range(M):
         t0 = time.time()
-        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, S_full)
+        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, sigma_z=site_f[k].result())
         Gh = runtime.to_host(G)
         print('mode: ', k, 'Total {:.3f}'.format(time.time() - t0))
         np.save(path + f"Gamma_{k}.npy", Gh)

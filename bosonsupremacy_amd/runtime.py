@@ -38,6 +38,8 @@ class Context:
     ordered with torch's allocations, fills, frees and copies of the same buffers."""
 
     def __init__(self, device=0):
+        from . import hostpool
+        hostpool.pool()  # fork the host workers before this process initialises the GPU
         _require_gpu()
         lib = _lib.load()
         self.device = torch.device("cuda", device)
@@ -97,6 +99,17 @@ def to_device(a, dtype=None, device=None):
     if dtype is not None:
         t = t.to(dtype)
     return t.to(device).contiguous()
+
+
+def upload(a, device=None):
+    """numpy -> device tensor without waiting for the stream: the array is staged in pinned
+    host memory (torch's caching host allocator keeps it alive until the copy has run) and
+    copied asynchronously on the current stream."""
+    device = device or context().device
+    arr = np.ascontiguousarray(a)
+    if arr.size == 0:
+        return torch.from_numpy(arr).to(device)
+    return torch.from_numpy(arr).pin_memory().to(device, non_blocking=True)
 
 
 def to_host(t):

@@ -59,7 +59,7 @@ int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
  * enough for one lane's registers to a one-lane-per-subtree kernel; results are identical to
  * the unsplit path (no reference counterpart). */
 int bsmps_set_split(bsmps_ctx *ctx, int n, int enable);
-/* Device scratch the split path may hold (default 1 GiB, at least 1 MiB): batches of
+/* Device scratch the split path may hold (default 4 GiB, at least 1 MiB): batches of
  * hafnians are processed in chunks whose frontier data fits it. */
 int bsmps_set_split_budget(bsmps_ctx *ctx, size_t bytes);
 

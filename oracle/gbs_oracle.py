@@ -51,6 +51,28 @@ from scipy.linalg import sqrtm, svd, block_diag, schur, expm
 # ---------------------------------------------------------------------------------------
 
 
+try:
+    from threadpoolctl import ThreadpoolController as _TC
+    _BLAS = _TC()
+except Exception:  # pragma: no cover
+    _BLAS = None
+
+
+def _single_thread_blas(fn):
+    """Single-threaded BLAS for the host decompositions: OpenBLAS results at these sizes
+    depend on its thread count; one thread is the reproducible configuration (the product
+    code pins the same)."""
+    import functools
+
+    @functools.wraps(fn)
+    def wrapper(*args, **kwargs):
+        if _BLAS is None:
+            return fn(*args, **kwargs)
+        with _BLAS.limit(limits=1, user_api="blas"):
+            return fn(*args, **kwargs)
+    return wrapper
+
+
 def sympmat(N, dtype=np.float64):
     """kron_cpu.py:23-27."""
     I = np.identity(N, dtype=dtype)
@@ -73,6 +95,7 @@ def xpxp_to_xxpp(S):
     return S[ind]
 
 
+@_single_thread_blas
 def williamson(V, tol=1e-11):
     """kron_cpu.py:46-89 (identical copy at MPS_cpu.py:72-115)."""
     (n, m) = V.shape
@@ -116,6 +139,7 @@ def thermal_photons(nth, cutoff=20):
     return 1 / (nth + 1) * (nth / (nth + 1)) ** np.arange(cutoff)
 
 
+@_single_thread_blas
 def blochmessiah(S):
     """MPS_cpu.py:189-229."""
     N, _ = S.shape
@@ -153,6 +177,7 @@ def blochmessiah(S):
     return uff, dff, vff
 
 
+@_single_thread_blas
 def get_U2_sq_U1(S_l, S_r):
     """MPS_cpu.py:357-372."""
     M = len(S_r) // 2
@@ -168,6 +193,7 @@ def get_U2_sq_U1(S_l, S_r):
     return U2, sq, U1
 
 
+@_single_thread_blas
 def get_Sigma(U2, sq, U1):
     """MPS_cpu.py:305-312, in complex128."""
     M = len(sq)
@@ -209,6 +235,7 @@ def kron_from_thermal(tps, chi, max_dim=10 ** 5, err_tol=10 ** (-12)):
     return np.ascontiguousarray(res), np.ascontiguousarray(num.astype('int8'))
 
 
+@_single_thread_blas
 def cut_thermal(sq_cov, L, cutoff):
     """Williamson of modes L..M-1 and the thermal distribution of each normal mode
     (kron_cpu.py:96-106)."""
@@ -374,6 +401,7 @@ def A_elem(Sigma, target, denominator, max_memory_in_gb=0.5):
 # ---------------------------------------------------------------------------------------
 
 
+@_single_thread_blas
 def site_sigma(compute_site, M, sq_cov, kron_S):
     """Sigma, sq and Z of one site, as MPS_cpu.py:395, 409-414, 439-442, 481-483 compute them."""
     if compute_site == 0:
