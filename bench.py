@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--workload", default="m32_d6_chi512")
     ap.add_argument("--samples", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-cutoff form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
@@ -276,6 +277,50 @@ def main():
         forms[prec]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
         forms[prec]["speedup_vs_fp64"] = forms[prec]["samples_per_s"] / forms["fp64"]["samples_per_s"]
 
+    # learned-cutoff form (per-site cutoffs and ranks from the MLP, each checked exactly) with
+    # the FP32 contraction, against the plain FP64 form on the same random numbers
+    learned_form = None
+    if not args.no_learned:
+        from bosonsupremacy_amd import learned
+        t0 = time.perf_counter()
+        model = learned.load_or_train()
+        t_train = time.perf_counter() - t0
+
+        def learned_construction():
+            t0 = time.perf_counter()
+            ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+            sd, sc, rep = learned.plan_sites(model, inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
+            G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=sd, site_chi=sc)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, G2, L2, rep
+
+        learned_construction()
+        t_lc, G2, L2, rep = learned_construction()
+        s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision="fp32")
+        s2.run(normals, uniforms)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s2.run(normals, uniforms)
+        torch.cuda.synchronize()
+        t_ls = time.perf_counter() - t0
+        _, probs = s2.run(normals, uniforms, return_probs=True)
+        rb_l = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
+        del probs
+        rel = float((torch.abs(rb_l - rb["fp64"]) / torch.abs(rb["fp64"])).max())
+        t_plain_samp = n_samples / forms["fp64"]["samples_per_s"]
+        learned_form = dict(rep)
+        learned_form.update({
+            "precision": "fp32 contraction",
+            "training_s": t_train,
+            "construction_s": t_lc,
+            "construction_speedup_vs_plain": t_build / t_lc,
+            "samples_per_s": n_samples / t_ls,
+            "sampling_speedup_vs_plain_fp64": t_plain_samp / t_ls,
+            "end_to_end_speedup_vs_plain_fp64": (t_build + t_plain_samp) / (t_lc + t_ls),
+            "first_order_marginal_max_rel_err_vs_plain_fp64": rel,
+        })
+        del G2, L2, s2
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(inst, kron_out)
@@ -305,6 +350,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "sampling_forms": forms,
+        "learned_form": learned_form,
         "tuning": {str(k): v for k, v in tuning.items()},
         "cpu_baseline": cpu,
     }

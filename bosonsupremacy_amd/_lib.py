@@ -21,6 +21,7 @@ c_int, c_int64, c_uint64, c_uint32, c_double = (ctypes.c_int, ctypes.c_int64, ct
                                                ctypes.c_uint32, ctypes.c_double)
 c_void_p = ctypes.c_void_p
 c_size_t = ctypes.c_size_t
+c_float = ctypes.c_float
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
 
@@ -76,6 +77,12 @@ _PROTOTYPES = {
     "bsmps_site_contract_bf16x3": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p], c_int),
     "bsmps_philox": ([c_void_p, c_uint64, c_uint32, c_int, c_int64, c_int64, c_int, c_void_p], c_int),
+    "bsmps_mlp_forward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 10, c_int),
+    "bsmps_mlp_backward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 13, c_int),
+    "bsmps_f32_to_bf16": ([c_void_p, c_int64, c_void_p, c_void_p], c_int),
+    "bsmps_adam_step": ([c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                         c_float, c_int], c_int),
+    "bsmps_mse_grad": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
 }
 
 EXPORTED_SYMBOLS = tuple(_PROTOTYPES)

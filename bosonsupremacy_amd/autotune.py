@@ -2,12 +2,12 @@
 
 For every even hafnian size it times the candidate launch shapes -- one lane per matrix
 (sizes <= 10), 16 / 32 / 64 cooperating lanes per matrix, and for sizes 12..20 the
-frontier-split form of each (cooperating lanes for the upper recursion tree, one lane per
-subtree below) -- on synthetic targets drawn from a random Sigma, and keeps the fastest.
+frontier-split form (upper recursion tree breadth-first in HBM, one lane per subtree below)
+-- on synthetic targets drawn from a random Sigma, and keeps the fastest.
 Every candidate computes bit-identical results (the tests check all shapes against the same
 oracle), so the choice only affects speed.  Results are cached per device name in
 ``~/.cache/bsmps/tuning.json`` (or $BSMPS_TUNING_CACHE) and applied to the process-wide
-context.  A choice is "lane", "group:G" or "split:G".
+context.  A choice is "lane", "group:G" or "split".
 """
 import json
 import os
@@ -20,7 +20,7 @@ from . import runtime
 GROUP_CANDIDATES = (16, 32, 64)
 LANE_MAX_SIZE = 10
 SPLIT_SIZES = range(12, 21, 2)
-CACHE_VERSION = "v2"
+CACHE_VERSION = "v3"
 
 
 def _cache_path():
@@ -42,7 +42,7 @@ def _time_call(fn, reps=2):
 
 
 def tune(sizes=(6, 8, 10, 12, 14, 16, 18), ld=64, items=None, seed=0, verbose=False):
-    """Time every candidate per size; returns {n: "lane" | "group:G" | "split:G"}."""
+    """Time every candidate per size; returns {n: "lane" | "group:G" | "split"}."""
     ctx = runtime.context()
     dev = ctx.device
     rng = np.random.default_rng(seed)
@@ -62,16 +62,16 @@ def tune(sizes=(6, 8, 10, 12, 14, 16, 18), ld=64, items=None, seed=0, verbose=Fa
             ctx.set_tuning(n // 2, 64)
             timings["lane"] = _time_call(run)
         ctx.set_tuning(min(2, n // 2 - 1) if n >= 6 else 2, 64)
-        forms = ("group", "split") if n in SPLIT_SIZES else ("group",)
-        for form in forms:
-            if n in SPLIT_SIZES:
-                ctx.call("bsmps_set_split", n, int(form == "split"))
-            for g in GROUP_CANDIDATES:
-                try:
-                    ctx.call("bsmps_set_group_lanes", n, g)
-                    timings[f"{form}:{g}"] = _time_call(run)
-                except Exception:
-                    continue
+        if n in SPLIT_SIZES:
+            ctx.call("bsmps_set_split", n, 1)
+            timings["split"] = _time_call(run)
+            ctx.call("bsmps_set_split", n, 0)
+        for g in GROUP_CANDIDATES:
+            try:
+                ctx.call("bsmps_set_group_lanes", n, g)
+                timings[f"group:{g}"] = _time_call(run)
+            except Exception:
+                continue
         best[n] = min(timings, key=timings.get)
         if verbose:
             print(f"n={n} items={count} " + " ".join(f"{k}:{v:.2f}ms" for k, v in timings.items()), flush=True)
@@ -79,7 +79,7 @@ def tune(sizes=(6, 8, 10, 12, 14, 16, 18), ld=64, items=None, seed=0, verbose=Fa
 
 
 def apply(best):
-    """Install a tuning {n: "lane" | "group:G" | "split:G"} on the process-wide context."""
+    """Install a tuning {n: "lane" | "group:G" | "split"} on the process-wide context."""
     ctx = runtime.context()
     lane_sizes = [n for n, v in best.items() if v == "lane"]
     lane_max_m = 2
@@ -88,12 +88,13 @@ def apply(best):
             lane_max_m = max(lane_max_m, n // 2)
     ctx.set_tuning(lane_max_m, 64)
     for n, v in best.items():
+        n = int(n)
         if v == "lane":
             continue
-        form, g = v.split(":")
-        ctx.call("bsmps_set_group_lanes", int(n), int(g))
-        if int(n) in SPLIT_SIZES:
-            ctx.call("bsmps_set_split", int(n), int(form == "split"))
+        if n in SPLIT_SIZES:
+            ctx.call("bsmps_set_split", n, int(v == "split"))
+        if v.startswith("group:"):
+            ctx.call("bsmps_set_group_lanes", n, int(v.split(":")[1]))
 
 
 def load_or_tune(verbose=False):

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libbsmps.so")
-SOURCES = ["context.hip", "hafnian.hip", "kron.hip", "sampling.hip"]
+SOURCES = ["context.hip", "hafnian.hip", "kron.hip", "sampling.hip", "mlp.hip"]
 OFFLOAD_ARCH = "gfx950"
 
 HIPCC_FLAGS = [

@@ -7,15 +7,21 @@
 // and leaves whose polynomial cannot reach degree n/2) are skipped: adding them only ever
 // adds +-0, which leaves every non-zero value unchanged.
 //
-// Two launch shapes:
+// Three launch shapes:
 //   * lane path   (n <= 10): one lane per matrix; the recursion is unrolled at compile time
 //                 and the polynomial stack lives in VGPRs.  The gathered matrix is staged
 //                 in LDS, laid out [entry][lane] so each wave reads it conflict-free.
 //   * group path  (n >= 6):  G = 16/32/64 lanes per matrix follow a precomputed depth-first
 //                 schedule of the recursion tree; each node's child matrix is built with one
 //                 lane per entry, the polynomial stack lives in LDS.
-// The matrices never exist in HBM: each lane gathers its entries from Sigma (L2-resident)
-// through its target indices.
+//   * split path  (n = 12..20): the upper recursion tree breadth-first in HBM, one launch
+//                 per level and one lane per (matrix, child entry); then one lane per
+//                 (matrix, subtree) for every subtree that fits one lane's registers; then a
+//                 tree-order combine.
+// The lane and group paths gather each matrix from Sigma (L2-resident) through its target
+// indices; the split path gathers it once into its scratch.
+//
+// Coefficients that cannot reach the result (live_degree below) are never computed.
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -1159,11 +1165,26 @@ extern "C" int bsmps_gamma_site(bsmps_ctx* ctx, const bsmps_gamma_site_args* a, 
         bsmps_set_error("bsmps_gamma_site: null argument");
         return BSMPS_ERR_INVALID;
     }
-    if (a->kind < 0 || a->kind > 2 || a->d <= 0 || a->chi <= 0 || a->ld <= 0 || a->n_left < 0 ||
-        a->n_right < 0 || a->n_left > a->chi || a->n_right > a->chi || a->n_left_class <= 0 ||
-        a->n_right_class <= 0 || !a->sigma || !a->sqrt_fact || !a->left_den || !a->lambda || !a->left_perm ||
-        !a->right_den || !a->right_perm || !a->left_class_count_host || !a->right_class_count_host) {
-        bsmps_set_error("bsmps_gamma_site: invalid arguments");
+    if (a->kind < 0 || a->kind > 2 || a->d <= 0 || a->chi <= 0 || a->ld <= 0 || a->n_left < 0 || a->n_right < 0) {
+        bsmps_set_error("bsmps_gamma_site: invalid sizes (kind %d, d %d, chi %d, ld %d, n_left %d, n_right %d)",
+                        a->kind, a->d, a->chi, a->ld, a->n_left, a->n_right);
+        return BSMPS_ERR_INVALID;
+    }
+    if (a->n_left > a->chi) {
+        bsmps_set_error("bsmps_gamma_site: %d left labels exceed chi = %d", a->n_left, a->chi);
+        return BSMPS_ERR_INVALID;
+    }
+    if (a->n_left_class <= 0 || a->n_right_class <= 0) {
+        bsmps_set_error("bsmps_gamma_site: empty photon classes (%d left, %d right)", a->n_left_class,
+                        a->n_right_class);
+        return BSMPS_ERR_INVALID;
+    }
+    if (!a->sigma || !a->sqrt_fact || !a->left_den || !a->lambda || !a->left_perm || !a->right_den ||
+        !a->right_perm || !a->left_class_count_host || !a->right_class_count_host) {
+        bsmps_set_error("bsmps_gamma_site: null pointer (sigma %p sqrt_fact %p left_den %p lambda %p left_perm %p "
+                        "right_den %p right_perm %p)", (const void*)a->sigma, (const void*)a->sqrt_fact,
+                        (const void*)a->left_den, (const void*)a->lambda, (const void*)a->left_perm,
+                        (const void*)a->right_den, (const void*)a->right_perm);
         return BSMPS_ERR_INVALID;
     }
     const int nlc = a->n_left_class, nrc = a->n_right_class;

@@ -117,6 +117,40 @@ def cut_inputs(sq_cov, L, cutoff):
     return tp, S
 
 
+def photon_distribution(V, nmax):
+    """Photon-number distribution p(0..nmax-1) of a zero-mean single-mode Gaussian state with
+    covariance V (2 x 2, (x, p) order, hbar = 2).
+
+    p(n) = haf(A_(n,n)) / (n! sqrt(det sigma_Q)) with sigma_Q = (V + I) / 2 in the (a, a^+)
+    basis and A = X (I - sigma_Q^-1) (the Gaussian-state Fock-amplitude formula of Hamilton et
+    al., PRL 119, 170501 (2017)).  For one mode, A = [[a, b], [b, a*]] and the hafnian of the
+    matrix with each row repeated n times is a sum over the number k of within-row pairs:
+    haf = sum_k (C(n, 2k) (2k-1)!!)^2 (n-2k)! |a|^(2k) b^(n-2k)."""
+    from math import comb, factorial
+    R = np.array([[1, 1j], [1, -1j]]) / np.sqrt(2)
+    sQ = R @ ((np.asarray(V, dtype=float) + np.eye(2)) / 2) @ R.conj().T
+    A = np.array([[0, 1], [1, 0]]) @ (np.eye(2) - np.linalg.inv(sQ))
+    a2, b = abs(A[0, 0]) ** 2, A[0, 1]
+    norm = 1.0 / np.sqrt(np.linalg.det(sQ).real)
+    p = np.zeros(nmax)
+    for n in range(nmax):
+        h = 0j
+        for k in range(n // 2 + 1):
+            dfact = 1
+            for i in range(1, 2 * k, 2):
+                dfact *= i
+            h += float((comb(n, 2 * k) * dfact) ** 2 * factorial(n - 2 * k)) * a2 ** k * b ** (n - 2 * k)
+        p[n] = (h / factorial(n)).real * norm
+    return p
+
+
+def photon_tail(V, d):
+    """P(n >= d) for the single-mode state of covariance V (see photon_distribution)."""
+    if d <= 0:
+        return 1.0
+    return float(max(0.0, 1.0 - np.sum(photon_distribution(V, d))))
+
+
 @single_thread_blas
 def blochmessiah(S):
     """Bloch-Messiah decomposition S = uff @ dff @ vff of a real symplectic S (xxpp)."""

@@ -151,32 +151,48 @@ def _dev_i32(a):
 
 
 def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None,
-                sigma_z=None):
+                sigma_z=None, ranks=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
 
     Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site.
-    ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel)."""
+    ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel).
+
+    ``ranks`` = (rank of cut k-1, rank of cut k) truncates the bond labels to the leading
+    Schmidt vectors (the learned-cutoff form; ``d`` is then this site's own cutoff): Gamma is
+    [rank_{k-1}][rank_k][d] and Lambda_k has rank_k entries (one row / one column at the
+    chain ends)."""
     ctx = runtime.context()
     dev = ctx.device
     Sigma, Z = sigma_z if sigma_z is not None else gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
     Mp = Sigma.shape[0] // 2
-    Lambda = np.zeros(chi)
+    r_prev, r_cur = ranks if ranks is not None else (None, None)
     if k == M - 1:
         kind = _lib.SITE_LAST
         left = _Labels.empty()
         lam = np.ones(1)
+        n_a = chi if ranks is None else 1
+        Lambda = np.zeros(n_a)
     else:
         kind = _lib.SITE_FIRST if k == 0 else _lib.SITE_MIDDLE
         res = np.asarray(kron_res[k], dtype=np.float64)
         keep = res > err_tol
-        left = _Labels(np.asarray(kron_num[k]).reshape(len(res), -1)[keep], 0)
+        num_k = np.asarray(kron_num[k]).reshape(len(res), -1)[keep]
         lam = np.sqrt(res[keep])
+        if r_cur is not None:
+            num_k, lam = num_k[:r_cur], lam[:r_cur]
+        left = _Labels(num_k, 0)
+        n_a = chi if ranks is None else len(lam)
+        Lambda = np.zeros(n_a)
         Lambda[:len(lam)] = lam
     if k == 0:
         right = _Labels.empty()
     else:
         num_pre = np.asarray(kron_num[k - 1])
-        right = _Labels(num_pre.reshape(num_pre.shape[0], -1), Mp - 1)
+        num_pre = num_pre.reshape(num_pre.shape[0], -1)
+        if r_prev is not None:
+            num_pre = num_pre[:r_prev]
+        right = _Labels(num_pre, Mp - 1)
+    n_b = chi if ranks is None else len(right.count)
     sqrt_fact = np.array([np.sqrt(factorial(j)) for j in np.arange(d)], dtype=np.float64)
 
     # uploads are asynchronous, so this host work overlaps the previous site's kernels
@@ -188,7 +204,7 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
     lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
     rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
     args = _lib.GammaSiteArgs(
-        kind=kind, d=d, chi=chi, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
+        kind=kind, d=d, chi=n_a, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
         sqrt_fact=sf_d.data_ptr(),
         n_left=len(left.count), w_left=left.targets.shape[1], left_targets=lt_d.data_ptr(),
         left_den=lden_d.data_ptr(), lambda_=lam_d.data_ptr(), left_perm=lp_d.data_ptr(),
@@ -196,16 +212,19 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
         n_right=len(right.count), w_right=right.targets.shape[1], right_targets=rt_d.data_ptr(),
         right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
         right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
-    gamma = torch.zeros((chi, chi, d), dtype=torch.complex128, device=dev)
+    gamma = torch.zeros((n_b, n_a, d), dtype=torch.complex128, device=dev)
     with (timer("gamma_site") if timer else contextlib.nullcontext()):
         ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
     return gamma, Lambda
 
 
-def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None):
+def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=None, site_chi=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
     outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them; when it is
     None the spectra are computed too.
+
+    ``site_d`` (per-site photon cutoffs <= d) and ``site_chi`` (per-cut ranks) select the
+    learned-cutoff form (bosonsupremacy_amd.learned.plan_sites); both None is the plain method.
 
     Pipelined: the host FP64 work of every site (Bloch-Messiah, Sigma) is queued on the
     host worker pool as soon as the cut symplectics are known -- before the Schmidt spectra
@@ -224,8 +243,18 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None):
         site_f = hostpool.submit_sites(sq_cov, kron[2], S_full_f.result())
     kres, knum, kS = kron
     Gammas, Lambdas = [], []
+    learned = site_d is not None or site_chi is not None
+    if learned:
+        site_d = list(site_d) if site_d is not None else [d] * M
+        if site_chi is None:
+            site_chi = [int((np.asarray(r) > ERR_TOL).sum()) for r in kres]
     for k in range(M):
-        G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result())
+        if learned:
+            ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
+            G, L = site_tensor(k, M, sq_cov, kres, knum, kS, site_d[k], chi, timer=timer,
+                               sigma_z=site_f[k].result(), ranks=ranks)
+        else:
+            G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result())
         Gammas.append(G)
         Lambdas.append(L)
     return Gammas, Lambdas, kron

@@ -93,8 +93,9 @@ def random_inputs(M, n, seed, offset=0):
 class Sampler:
     """Samples photon numbers from an MPS held as device tensors.
 
-    gammas : list of M device tensors [chi, chi, d] complex128
-    lambdas: list of M float64 arrays [chi] (the last one unused)
+    gammas : list of M device tensors [chi_{i-1}, chi_i, d_i] complex128 (the plain method:
+             all [chi, chi, d]; site 0 uses row 0 only, so its first extent may be 1)
+    lambdas: list of M float64 arrays [chi_i] (the last one unused)
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
     """
 
@@ -104,10 +105,18 @@ class Sampler:
         self.ctx = runtime.context()
         dev = self.ctx.device
         self.M = len(gammas)
-        self.chi, _, self.d = gammas[0].shape
+        self.chis = [int(g.shape[1]) for g in gammas]
+        self.ds = [int(g.shape[2]) for g in gammas]
+        self.chi, self.d = max(self.chis), max(self.ds)
         self.dd = dd
         if dd < self.d:
             raise ValueError("dd must be >= d")
+        for i in range(1, self.M):
+            if gammas[i].shape[0] < self.chis[i - 1]:
+                raise ValueError(f"Gamma_{i} has {gammas[i].shape[0]} rows, fewer than the {self.chis[i - 1]} "
+                                 f"Schmidt vectors of the previous cut")
+            if len(lambdas[i - 1]) < self.chis[i - 1]:
+                raise ValueError(f"Lambda_{i - 1} has fewer than {self.chis[i - 1]} entries")
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         self.precision = precision
@@ -136,7 +145,7 @@ class Sampler:
         context-manager factory ``timer(name)`` wrapped around each kernel launch."""
         import contextlib
         timer = timer or (lambda name: contextlib.nullcontext())
-        ctx, M, chi, d, dd = self.ctx, self.M, self.chi, self.d, self.dd
+        ctx, M, dd = self.ctx, self.M, self.dd
         n = normals.shape[1]
         normals = normals.contiguous()
         uniforms = uniforms.contiguous()
@@ -145,24 +154,26 @@ class Sampler:
         samples = torch.empty((n, M), dtype=torch.int32, device=self.dev)
         ctype = torch.complex128 if self.precision == "fp64" else torch.complex64
         draw = "bsmps_sample_site" if self.precision == "fp64" else "bsmps_sample_site_c64"
-        pre = torch.empty((n, chi), dtype=ctype, device=self.dev)
-        T = torch.empty((n, chi, d), dtype=ctype, device=self.dev)
+        pre = torch.empty((n * self.chi,), dtype=ctype, device=self.dev)
+        T = torch.empty((n * max(c * dk for c, dk in zip(self.chis, self.ds)),), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
         for i in range(M):
             last = i == M - 1
+            chi, d = self.chis[i], self.ds[i]
             if i == 0:
                 src, stride = self.gammas[0], 0  # Gamma_0[0, a, j] is the site-0 tensor for every sample
             else:
+                chi_p = self.chis[i - 1]
                 with timer("site_contract"):
                     if self.precision == "fp64":
-                        ctx.call("bsmps_site_contract", n, chi, chi, d, runtime.ptr(pre),
+                        ctx.call("bsmps_site_contract", n, chi_p, chi, d, runtime.ptr(pre),
                                  runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
                     elif self.precision == "fp32":
-                        ctx.call("bsmps_site_contract_c64", n, chi, chi, d, runtime.ptr(pre),
+                        ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre),
                                  runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
                     else:
                         hi, lo = self.splits[i]
-                        ctx.call("bsmps_site_contract_bf16x3", n, chi, chi, d, runtime.ptr(pre),
+                        ctx.call("bsmps_site_contract_bf16x3", n, chi_p, chi, d, runtime.ptr(pre),
                                  runtime.ptr(self.lambdas[i - 1]), runtime.ptr(hi), runtime.ptr(lo), runtime.ptr(T))
                 src, stride = T, chi * d
             a_ptr = ctypes.c_void_p(alpha.data_ptr() + 16 * i)

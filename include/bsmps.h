@@ -55,9 +55,10 @@ int bsmps_version(void);
 int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
 int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
 /* Hafnians of size n in 12..20 take the frontier-split path when `enable` is 1 (default): the
- * cooperative lanes run the upper part of the recursion tree and hand each subtree small
- * enough for one lane's registers to a one-lane-per-subtree kernel; results are identical to
- * the unsplit path (no reference counterpart). */
+ * upper part of the recursion tree runs breadth-first, one launch per tree level, with its
+ * polynomials in device scratch, and every subtree small enough for one lane's registers is
+ * evaluated by a one-lane-per-subtree kernel; results are identical to the unsplit path (no
+ * reference counterpart). */
 int bsmps_set_split(bsmps_ctx *ctx, int n, int enable);
 /* Device scratch the split path may hold (default 4 GiB, at least 1 MiB): batches of
  * hafnians are processed in chunks whose frontier data fits it. */
@@ -95,7 +96,7 @@ enum { BSMPS_SITE_FIRST = 0, BSMPS_SITE_MIDDLE = 1, BSMPS_SITE_LAST = 2 };
 typedef struct {
     int kind;                     /* BSMPS_SITE_* */
     int d;                        /* local cutoff: j = 0..d-1 */
-    int chi;                      /* Gamma is [chi][chi][d] */
+    int chi;                      /* Gamma is [n_right][chi][d] (the reference: [chi][chi][d]); n_left <= chi */
     int ld;                       /* Sigma is [ld][ld], ld = 2 M' */
     const double *sigma;          /* device complex128 [ld][ld] */
     double z;                     /* Z = sqrt(prod cosh(sq)), MPS_cpu.py:483 */
@@ -209,6 +210,32 @@ int bsmps_site_contract_bf16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi,
  * out device double [n_streams][n]. */
 int bsmps_philox(bsmps_ctx *ctx, uint64_t seed, uint32_t stream0, int n_streams, int64_t first, int64_t n,
                  int normal, double *out);
+
+/* ---- learned-cutoff form: a small MLP trained and evaluated on the GPU (no reference
+ * counterpart; the reference has no learned component) ------------------------------------ */
+
+/* Forward pass of the in -> width -> width -> out ReLU network:
+ *   H1 = relu(X W1^T + b1), H2 = relu(H1 W2^T + b2), Y = H2 W3^T + b3.
+ * X float [n][d_in]; W1 bf16 [width][d_in], W2 bf16 [width][width], W3 bf16 [d_out][width];
+ * biases float; H1, H2 float [n][width]; Y float [n][d_out].  Products on BF16 MFMA with FP32
+ * accumulation; FP32 activations are rounded to BF16 (nearest-even) as operands. */
+int bsmps_mlp_forward(bsmps_ctx *ctx, int n, int d_in, int width, int d_out, const float *X, const void *W1,
+                      const float *b1, const void *W2, const float *b2, const void *W3, const float *b3,
+                      float *H1, float *H2, float *Y);
+/* Backward pass for the loss gradient dY float [n][d_out] (H1, H2 from the forward pass):
+ * weight gradients float [fan_out][fan_in] and bias gradients float, summed over the n rows.
+ * scratch: float [2 n width]. */
+int bsmps_mlp_backward(bsmps_ctx *ctx, int n, int d_in, int width, int d_out, const float *X, const void *W2,
+                       const void *W3, const float *H1, const float *H2, const float *dY, float *scratch,
+                       float *dW1, float *db1, float *dW2, float *db2, float *dW3, float *db3);
+/* dst[i] = bf16(src[i]), round to nearest even. */
+int bsmps_f32_to_bf16(bsmps_ctx *ctx, int64_t count, const float *src, void *dst);
+/* One Adam step (Kingma & Ba, 2015) with bias correction for step >= 1:
+ * m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; p -= lr (m / (1-b1^step)) / (sqrt(v / (1-b2^step)) + eps). */
+int bsmps_adam_step(bsmps_ctx *ctx, int64_t count, float *param, const float *grad, float *m, float *v, float lr,
+                    float beta1, float beta2, float eps, int step);
+/* Mean squared error gradient over n x d_out: dY = 2 (Y - T) / (n d_out); err (optional) = (Y - T)^2. */
+int bsmps_mse_grad(bsmps_ctx *ctx, int n, int d_out, const float *Y, const float *T, float *dY, float *err);
 
 #ifdef __cplusplus
 }
