@@ -90,6 +90,21 @@ def site_hafnian_work(kres, knum, d, err_tol=1e-10):
     return total_n, total_f
 
 
+def measured_traffic(workload, n_samples):
+    """HBM bytes per site_contract launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs (profiles/<workload>_traffic.json, scaled by a 1 GiB copy), when they cover this
+    workload and sample count; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("n_samples") != n_samples:
+        return None
+    return 1024.0 * (t["fetch_kib_per_dispatch"] * t["fetch_factor"] + t["write_kib_per_dispatch"] * t["write_factor"])
+
+
 class KernelTimer:
     """HIP events on the library's stream (torch's current stream) around each launch."""
 
@@ -173,7 +188,23 @@ def main():
     ap.add_argument("--samples", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cutoff form")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="time the plain step only (no precision / learned forms, no CPU baseline)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 8, "copies": 3}))
+        return
+    if args.no_forms:
+        args.no_learned = True
+        args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
               f"single GPU with no collectives)", file=sys.stderr)
@@ -245,8 +276,9 @@ def main():
     if dom == "site_contract":
         ach = kernels[dom]["tflops"]
         roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
-                "algorithmic": f"8*n*chi^2*d = {gemm_flops:.3e} FP64 flops per launch"}
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.workload, n_samples),
+                "algorithmic": f"8*n*chi^2*d = {gemm_flops:.3e} FP64 flops per launch",
+                "algorithmic_bytes": 16.0 * (n_samples * chi + chi * chi * d + n_samples * chi * d)}
     elif dom == "gamma_site":
         ach = kernels[dom]["tflops"]
         roof = {"kernel": dom, "bound": "fp64-vector", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
@@ -260,7 +292,7 @@ def main():
     forms = {}
     normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=2)
     rb = {}
-    for prec in ("fp64", "fp32", "bf16x3"):
+    for prec in (() if args.no_forms else ("fp64", "fp32", "bf16x3")):
         s = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
         s.run(normals, uniforms)
         torch.cuda.synchronize()
@@ -272,7 +304,7 @@ def main():
         rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
         forms[prec] = {"samples_per_s": n_samples / dt}
         del probs
-    for prec in ("fp32", "bf16x3"):
+    for prec in (() if args.no_forms else ("fp32", "bf16x3")):
         rel = torch.abs(rb[prec] - rb["fp64"]) / torch.abs(rb["fp64"])
         forms[prec]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
         forms[prec]["speedup_vs_fp64"] = forms[prec]["samples_per_s"] / forms["fp64"]["samples_per_s"]
