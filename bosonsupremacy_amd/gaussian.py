@@ -139,8 +139,10 @@ def photon_distribution(V, nmax):
             dfact = 1
             for i in range(1, 2 * k, 2):
                 dfact *= i
-            h += float((comb(n, 2 * k) * dfact) ** 2 * factorial(n - 2 * k)) * a2 ** k * b ** (n - 2 * k)
-        p[n] = (h / factorial(n)).real * norm
+            # exact integer ratio, rounded once: (C(n,2k) (2k-1)!!)^2 (n-2k)! / n!
+            coef = (comb(n, 2 * k) * dfact) ** 2 * factorial(n - 2 * k) / factorial(n)
+            h += coef * a2 ** k * b ** (n - 2 * k)
+        p[n] = h.real * norm
     return p
 
 

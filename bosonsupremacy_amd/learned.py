@@ -33,7 +33,14 @@ from . import gaussian, runtime
 N_FEATURES = 16
 WIDTH = 128
 N_OUT = 2
-TAU_D = 1e-7     # extra photon weight a site may drop by its cutoff
+# Tolerances of the exact checks.  A dropped Schmidt vector changes the conditional
+# distributions linearly in its weight (bond indices are summed incoherently in p = sum_a |.|^2),
+# but a dropped local component j >= d_k is not orthogonal to the kept ones after the
+# displacement D(alpha) mixes photon numbers, so dropping photon weight eps moves the displaced
+# probabilities by O(sqrt(eps)) through interference (measured on the M=64 workload: 1e-7 of
+# dropped photon weight moved first-order marginals by 4e-5).  A 1e-6 marginal target
+# therefore allows ~1e-12 of photon weight but ~1e-7 of Schmidt weight.
+TAU_D = 1e-12    # extra photon weight a site may drop by its cutoff
 TAU_CHI = 1e-7   # extra Schmidt weight a cut may drop by its rank
 D_SCALE = 10.0   # target scaling: cutoff / 10, log2(rank fraction) / 10
 R_SCALE = 10.0
@@ -71,7 +78,8 @@ def mode_covariance(sq_cov, k):
 
 
 def needed_cutoff(sq_cov, k, d_cap, tau=TAU_D):
-    """Smallest cutoff d' (<= d_cap) with P(n_k >= d') <= tau in the pure state."""
+    """Smallest cutoff d' (<= d_cap) with P(n_k >= d') <= tau in the pure state (d_cap when
+    even d_cap does not reach tau)."""
     p = gaussian.photon_distribution(mode_covariance(sq_cov, k), d_cap)
     tail = 1.0 - np.cumsum(p)
     ok = np.nonzero(tail <= tau)[0]
@@ -216,8 +224,12 @@ class MLP:
         self._cast()
 
 
+MODEL_VERSION = 2  # bump when features, labels or tolerances change
+
+
 def _cache_path():
-    return os.environ.get("BSMPS_MLP_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "bsmps", "cutoff_mlp.npz"))
+    return os.environ.get("BSMPS_MLP_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "bsmps",
+                                                          f"cutoff_mlp_v{MODEL_VERSION}.npz"))
 
 
 TRAINING_WORKLOADS = ("m8_d4_chi64", "m32_d6_chi512", "m64_d8_chi2000")

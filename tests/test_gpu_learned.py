@@ -124,4 +124,4 @@ def test_learned_form_gain_on_unseen_instance():
     mb = (pb * np.arange(inst.dd)).sum(axis=2).mean(axis=1)
     rel = np.max(np.abs(ma - mb) / ma)
     print("first-order marginal max rel diff learned vs plain", rel, flush=True)
-    assert rel < 1e-4
+    assert rel < 1e-6
