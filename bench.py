@@ -64,6 +64,150 @@ def hafnian_flops(m):
     return rec(0, 0, 0) if m >= 3 else {0: 0, 1: 0, 2: 22}[m]
 
 
+def site_hafnian_work_by_site(kres, knum, d, err_tol=1e-10):
+    """[(hafnian count, flops, largest size)] of every site tensor, from the label photon counts."""
+    M = len(kres) + 1
+    out = []
+    fl = {}
+    for k in range(M):
+        if k == M - 1:
+            la = np.zeros(1, int)
+        else:
+            la = np.asarray(knum[k]).reshape(len(kres[k]), -1)[np.asarray(kres[k]) > err_tol].sum(1)
+        lb = np.zeros(1, int) if k == 0 else np.asarray(knum[k - 1]).reshape(len(knum[k - 1]), -1).sum(1)
+        ca, cb = np.bincount(la), np.bincount(lb)
+        cnt, flops, big = 0, 0.0, 0
+        for j in range(d):
+            for p in range(len(ca)):
+                for q in range(len(cb)):
+                    n = j + p + q
+                    if n % 2 or ca[p] == 0 or cb[q] == 0:
+                        continue
+                    c = int(ca[p]) * int(cb[q])
+                    if n // 2 not in fl:
+                        fl[n // 2] = hafnian_flops(n // 2)
+                    cnt += c
+                    flops += c * fl[n // 2]
+                    big = max(big, n)
+        out.append((cnt, flops, big))
+    return out
+
+
+def probe_sampling(inst, ko, probe, n):
+    """Time one site's sampling step at full size: the contraction of n carried vectors with
+    a real Gamma_k (FP64 and FP32 forms) and the displacement + draw, then extrapolate the
+    chain's samples/s as n / ((M - 1) x per-site time)."""
+    import torch
+    from bosonsupremacy_amd import runtime, sampling
+    ctx = runtime.context()
+    dev = ctx.device
+    k, g_host, lam_k = probe
+    chi_p, chi, d = g_host.shape
+    G = g_host.to(dev)
+    # Lambda vectors padded to the Gamma extents (the kernels read chi_p and chi entries)
+    res_prev = np.asarray(ko[0][k - 1], dtype=np.float64)[:chi_p]
+    lp = np.zeros(chi_p)
+    lp[:len(res_prev)] = np.sqrt(res_prev)
+    lam_prev = runtime.to_device(lp)
+    lk = np.zeros(chi)
+    lk[:min(chi, len(lam_k))] = np.asarray(lam_k, dtype=np.float64)[:chi]
+    lam = runtime.to_device(lk)
+    rng = torch.Generator(device=dev).manual_seed(0)
+    pre = torch.randn((n, chi_p), dtype=torch.complex128, device=dev, generator=rng)
+    pre /= torch.linalg.vector_norm(pre, dim=1, keepdim=True)
+    T = torch.empty((n, chi, d), dtype=torch.complex128, device=dev)
+
+    def timed(fn, reps=2):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps / 1e3
+
+    t64 = timed(lambda: ctx.call("bsmps_site_contract", n, chi_p, chi, d, runtime.ptr(pre), runtime.ptr(lam_prev),
+                                 runtime.ptr(G), runtime.ptr(T)))
+    G32 = G.to(torch.complex64)
+    del G
+    pre32 = pre.to(torch.complex64)
+    T32 = torch.empty((n, chi, d), dtype=torch.complex64, device=dev)
+    t32 = timed(lambda: ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre32),
+                                 runtime.ptr(lam_prev), runtime.ptr(G32), runtime.ptr(T32)))
+    W, x = sampling.displacement_basis(inst.dd)
+    Wd, xd = runtime.to_device(W), runtime.to_device(x)
+    alpha = (torch.randn((n, inst.M), dtype=torch.complex128, device=dev, generator=rng) * 0.3).contiguous()
+    u = torch.rand((n,), dtype=torch.float64, device=dev, generator=rng)
+    smp = torch.empty((n, 1), dtype=torch.int32, device=dev)
+    nxt = torch.empty((n, chi), dtype=torch.complex128, device=dev)
+    ts = timed(lambda: ctx.call("bsmps_sample_site", n, chi, d, inst.dd, runtime.ptr(T), chi * d,
+                                runtime.ptr(alpha), inst.M, runtime.ptr(Wd), runtime.ptr(xd), runtime.ptr(lam),
+                                runtime.ptr(u), runtime.ptr(smp), 1, runtime.ptr(nxt), None))
+    flops = 8.0 * n * chi_p * chi * d
+    out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_fp64_s": t64, "contract_fp32_s": t32,
+           "sample_site_s": ts, "contract_fp64_tflops": flops / t64 / 1e12, "contract_fp32_tflops": flops / t32 / 1e12,
+           "chain_samples_per_s_fp64": n / ((inst.M - 1) * (t64 + ts)),
+           "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts))}
+    log(f"sampling probe at site {k}: {out}")
+    return out
+
+
+def partial_run(args):
+    """Part of an instance too large for one session (the M=144, chi=10^4 workload): the
+    Schmidt spectra of every cut, the exact hafnian work of every site, and the listed sites
+    built at full size with Gamma streamed to pinned host memory; the whole chain's
+    construction time is extrapolated from the measured hafnian rate."""
+    import torch
+    from bosonsupremacy_amd import _lib, autotune, instances, kron, mps, runtime
+    inst = instances.workload(args.workload)
+    sites = [int(x) for x in args.partial_sites.split(",")]
+    runtime.context()
+    autotune.load_or_tune()
+    log("tuned")
+    t0 = time.perf_counter()
+    ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+    torch.cuda.synchronize()
+    t_kron = time.perf_counter() - t0
+    log(f"spectra of {inst.M - 1} cuts: {t_kron:.2f} s")
+    work = site_hafnian_work_by_site(ko[0], ko[1], inst.d)
+    rows = []
+    timer = KernelTimer()
+    timer.enabled = True
+    sample_probe = None
+    for k in sites:
+        if work[k][2] > _lib.MAX_HAFNIAN_SIZE:
+            rows.append({"site": k, "hafnians": work[k][0], "largest_size": work[k][2], "built": False})
+            continue
+        t0 = time.perf_counter()
+        G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, timer=timer, to_host=True, sites=[k])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        g = G[k]
+        rows.append({"site": k, "hafnians": work[k][0], "flops": work[k][1], "largest_size": work[k][2],
+                     "built": True, "seconds": dt, "tflops": work[k][1] / dt / 1e12,
+                     "gamma_shape": list(g.shape), "gamma_bytes": g.numel() * 16, "gamma_pinned": bool(g.is_pinned())})
+        log(f"site {k}: {work[k][0]:.3e} hafnians (largest {work[k][2]}), {dt:.2f} s")
+        if args.partial_sample and k == inst.M // 2:
+            sample_probe = (k, g, L[k])
+        del G, g
+    built = [r for r in rows if r["built"]]
+    rate = sum(r["flops"] for r in built) / sum(r["seconds"] for r in built)
+    supported = [w for w in work if w[2] <= _lib.MAX_HAFNIAN_SIZE]
+    sampling_rate = None
+    if sample_probe is not None:
+        sampling_rate = probe_sampling(inst, ko, sample_probe, args.partial_sample)
+    out = {"workload": args.workload, "partial": True, "M": inst.M, "d": inst.d, "chi": inst.chi,
+           "sampling_probe": sampling_rate,
+           "kron_s": t_kron, "sites": rows,
+           "measured_rate_tflops": rate / 1e12,
+           "total_hafnians": int(sum(w[0] for w in work)), "total_flops": float(sum(w[1] for w in work)),
+           "sites_beyond_max_size": [k for k, w in enumerate(work) if w[2] > _lib.MAX_HAFNIAN_SIZE],
+           "extrapolated_construction_s_supported_sites": t_kron + sum(w[1] for w in supported) / rate}
+    print(json.dumps(out))
+
+
 def site_hafnian_work(kres, knum, d, err_tol=1e-10):
     """(hafnian count, flops) of all site tensors, from the label photon counts."""
     M = len(kres) + 1
@@ -195,6 +339,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cutoff form")
     ap.add_argument("--no-forms", action="store_true",
                     help="time the plain step only (no precision / learned forms, no CPU baseline)")
+    ap.add_argument("--partial-sites", default=None,
+                    help="comma-separated sites: build only these (Gamma streamed to pinned host) and "
+                         "extrapolate the whole construction (for instances too large for one run)")
+    ap.add_argument("--partial-sample", type=int, default=0,
+                    help="with --partial-sites including M//2: time that site's sampling step for this many samples")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -210,6 +359,9 @@ def main():
     if args.no_forms:
         args.no_learned = True
         args.no_cpu_baseline = True
+    if args.partial_sites:
+        partial_run(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
               f"single GPU with no collectives)", file=sys.stderr)

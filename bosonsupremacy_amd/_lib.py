@@ -13,7 +13,7 @@ BSMPS_OK = 0
 BSMPS_ERR_INVALID = 1
 BSMPS_ERR_HIP = 2
 BSMPS_ERR_UNSUPPORTED = 3
-MAX_HAFNIAN_SIZE = 24
+MAX_HAFNIAN_SIZE = 26
 
 SITE_FIRST, SITE_MIDDLE, SITE_LAST = 0, 1, 2
 

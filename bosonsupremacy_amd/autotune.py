@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the hafnian kernels.
 
 For every even hafnian size it times the candidate launch shapes -- one lane per matrix
-(sizes <= 10), 16 / 32 / 64 cooperating lanes per matrix, and for sizes 12..24 the
+(sizes <= 10), 16 / 32 / 64 cooperating lanes per matrix, and for sizes 12..26 the
 frontier-split form (upper recursion tree breadth-first in HBM, one lane per subtree below)
 -- on synthetic targets drawn from a random Sigma, and keeps the fastest.
 Every candidate computes bit-identical results (the tests check all shapes against the same
@@ -19,7 +19,7 @@ from . import runtime
 
 GROUP_CANDIDATES = (16, 32, 64)
 LANE_MAX_SIZE = 10
-SPLIT_SIZES = range(12, 25, 2)
+SPLIT_SIZES = range(12, 27, 2)
 CACHE_VERSION = "v4"
 
 

@@ -57,7 +57,7 @@ struct bsmps_ctx {
     int lane_max_m;   // hafnians of size <= 2*lane_max_m take the one-lane-per-hafnian path
     int group_lanes_by_m[16];  // lanes per hafnian in the cooperative path (16, 32 or 64), by n/2
     int max_lds;      // bytes of LDS one workgroup may use on this device
-    int split_by_m[16];  // 1: hafnians of size 2m take the frontier-split path (n = 12..24)
+    int split_by_m[16];  // 1: hafnians of size 2m take the frontier-split path (n = 12..26)
     // device scratch of the frontier-split hafnian path (separate from `ws`, which holds the
     // per-call tables that are live while the hafnian launches run)
     void* hws;

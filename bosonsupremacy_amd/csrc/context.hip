@@ -95,7 +95,7 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     c->ws_bytes = 0;
     c->lane_max_m = 4;
     for (int m = 0; m < 16; ++m) c->group_lanes_by_m[m] = 64;
-    for (int m = 0; m < 16; ++m) c->split_by_m[m] = (m >= 6 && m <= 12) ? 1 : 0;
+    for (int m = 0; m < 16; ++m) c->split_by_m[m] = (m >= 6 && m <= 13) ? 1 : 0;
     c->hws = nullptr;
     c->hws_bytes = 0;
     c->split_chunk_bytes = (size_t)4 << 30;
@@ -166,8 +166,8 @@ extern "C" int bsmps_set_group_lanes(bsmps_ctx* ctx, int n, int group_lanes) {
 }
 
 extern "C" int bsmps_set_split(bsmps_ctx* ctx, int n, int enable) {
-    if (!ctx || n < 12 || n > 24 || n % 2 || (enable != 0 && enable != 1)) {
-        bsmps_set_error("bsmps_set_split: n must be even in 12..24 and enable 0 or 1");
+    if (!ctx || n < 12 || n > 26 || n % 2 || (enable != 0 && enable != 1)) {
+        bsmps_set_error("bsmps_set_split: n must be even in 12..26 and enable 0 or 1");
         return BSMPS_ERR_INVALID;
     }
     ctx->split_by_m[n / 2] = enable;

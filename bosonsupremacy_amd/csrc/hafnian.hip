@@ -14,7 +14,7 @@
 //   * group path  (n >= 6):  G = 16/32/64 lanes per matrix follow a precomputed depth-first
 //                 schedule of the recursion tree; each node's child matrix is built with one
 //                 lane per entry, the polynomial stack lives in LDS.
-//   * split path  (n = 12..24): the upper recursion tree breadth-first in HBM, one launch
+//   * split path  (n = 12..26): the upper recursion tree breadth-first in HBM, one launch
 //                 per level and one lane per (matrix, child entry); then one lane per
 //                 (matrix, subtree) for every subtree that fits one lane's registers; then a
 //                 tree-order combine.
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Frontier split (sizes 12..24).  The recursion tree is cut at "frontier" nodes whose subtree
+// Frontier split (sizes 12..26).  The recursion tree is cut at "frontier" nodes whose subtree
 // fits one lane's registers.  The part above the frontier runs breadth-first, one launch per
 // tree level, with one lane per (hafnian, child-matrix entry): every node that owns new
 // polynomials (the root and each right child) keeps them in HBM, laid out
@@ -855,7 +855,7 @@ int launch_group(bsmps_ctx* ctx, const S& src, int m) {
 #undef BSMPS_GROUP_CASE
         default:
             bsmps_set_error("group hafnian path supports sizes 6..24, got %d", 2 * m);
-            return BSMPS_ERR_INVALID;
+            return BSMPS_ERR_UNSUPPORTED;
     }
     return BSMPS_OK;
 }
@@ -897,8 +897,8 @@ struct BfsBuilder {
 };
 
 // Subtree shapes with a compiled one-lane kernel: (K0, D) = (M-4, 0), (M-4, 1), (M-3, 3),
-// (M-2, 7) and (M-2, M), for M = 6..12.
-constexpr int kSplitMinM = 6, kSplitMaxM = 12;
+// (M-2, 7) and (M-2, M), for M = 6..13.
+constexpr int kSplitMinM = 6, kSplitMaxM = 13;
 bool subtree_compiled(int M, int K0, int D) {
     if (M < kSplitMinM || M > kSplitMaxM) return false;
     return (K0 == M - 4 && (D == 0 || D == 1)) || (K0 == M - 3 && D == 3) ||
@@ -1052,9 +1052,10 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
             BSMPS_SPLIT_CASE(10)
             BSMPS_SPLIT_CASE(11)
             BSMPS_SPLIT_CASE(12)
+            BSMPS_SPLIT_CASE(13)
 #undef BSMPS_SPLIT_CASE
             default:
-                bsmps_set_error("frontier-split hafnian path supports sizes 12..24, got %d", 2 * m);
+                bsmps_set_error("frontier-split hafnian path supports sizes 12..26, got %d", 2 * m);
                 return BSMPS_ERR_INVALID;
         }
         hipLaunchKernelGGL((haf_combine_kernel<S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, nh, H, vals,

@@ -218,13 +218,49 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
     return gamma, Lambda
 
 
-def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=None, site_chi=None):
+class HostStream:
+    """Streams device tensors to pinned host memory on a copy stream (default priority),
+    overlapping the compute stream: each copy starts when the compute stream has produced the
+    tensor, the device memory is released to the allocator once its copy has run, and at most
+    ``depth`` copies are in flight (which bounds the device memory held by finished tensors)."""
+
+    def __init__(self, depth=2):
+        self.stream = torch.cuda.Stream()
+        self.depth = depth
+        self.pending = []
+
+    def push(self, t):
+        ready = torch.cuda.Event()
+        ready.record()
+        host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        with torch.cuda.stream(self.stream):
+            self.stream.wait_event(ready)
+            host.copy_(t, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(self.stream)
+        t.record_stream(self.stream)
+        self.pending.append(done)
+        while len(self.pending) > self.depth:
+            self.pending.pop(0).synchronize()
+        return host
+
+    def drain(self):
+        for ev in self.pending:
+            ev.synchronize()
+        self.pending = []
+
+
+def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=None, site_chi=None,
+              to_host=False, sites=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
     outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them; when it is
     None the spectra are computed too.
 
     ``site_d`` (per-site photon cutoffs <= d) and ``site_chi`` (per-cut ranks) select the
     learned-cutoff form (bosonsupremacy_amd.learned.plan_sites); both None is the plain method.
+    ``to_host`` streams every Gamma_k to pinned host memory as soon as it is computed (the
+    returned Gammas are then pinned host tensors and the device copies are released);
+    ``sites`` builds only the listed sites (None entries elsewhere).
 
     Pipelined: the host FP64 work of every site (Bloch-Messiah, Sigma) is queued on the
     host worker pool as soon as the cut symplectics are known -- before the Schmidt spectra
@@ -248,15 +284,25 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
         site_d = list(site_d) if site_d is not None else [d] * M
         if site_chi is None:
             site_chi = [int((np.asarray(r) > ERR_TOL).sum()) for r in kres]
+    streamer = HostStream() if to_host else None
+    wanted = set(range(M)) if sites is None else set(sites)
     for k in range(M):
+        if k not in wanted:
+            Gammas.append(None)
+            Lambdas.append(None)
+            continue
         if learned:
             ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, site_d[k], chi, timer=timer,
                                sigma_z=site_f[k].result(), ranks=ranks)
         else:
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result())
+        if streamer is not None:
+            G = streamer.push(G)
         Gammas.append(G)
         Lambdas.append(L)
+    if streamer is not None:
+        streamer.drain()
     return Gammas, Lambdas, kron
 
 

@@ -32,7 +32,7 @@ enum {
 };
 
 /* Largest hafnian matrix size (rows) the hafnian entry points accept. */
-#define BSMPS_MAX_HAFNIAN_SIZE 24
+#define BSMPS_MAX_HAFNIAN_SIZE 26
 
 typedef struct bsmps_ctx bsmps_ctx;
 
@@ -54,7 +54,7 @@ int bsmps_version(void);
  * for every size (bsmps_set_tuning) or for size n only (bsmps_set_group_lanes). */
 int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
 int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
-/* Hafnians of size n in 12..24 take the frontier-split path when `enable` is 1 (default): the
+/* Hafnians of size n in 12..26 take the frontier-split path when `enable` is 1 (default): the
  * upper part of the recursion tree runs breadth-first, one launch per tree level, with its
  * polynomials in device scratch, and every subtree small enough for one lane's registers is
  * evaluated by a one-lane-per-subtree kernel; results are identical to the unsplit path (no
