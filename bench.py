@@ -93,7 +93,7 @@ def site_hafnian_work_by_site(kres, knum, d, err_tol=1e-10):
     return out
 
 
-def probe_sampling(inst, ko, probe, n):
+def probe_sampling(inst, ko, probe, n, tuning):
     """Time one site's sampling step at full size: the contraction of n carried vectors with
     a real Gamma_k (FP64 and FP32 forms) and the displacement + draw, then extrapolate the
     chain's samples/s as n / ((M - 1) x per-site time)."""
@@ -145,9 +145,12 @@ def probe_sampling(inst, ko, probe, n):
     ts = timed(lambda: ctx.call("bsmps_sample_site", n, chi, d, inst.dd, runtime.ptr(T), chi * d,
                                 runtime.ptr(alpha), inst.M, runtime.ptr(Wd), runtime.ptr(xd), runtime.ptr(lam),
                                 runtime.ptr(u), runtime.ptr(smp), 1, runtime.ptr(nxt), None))
-    flops = 8.0 * n * chi_p * chi * d
+    forms = {k: tuning.get(k, "blocks") for k in ("contract_fp64", "contract_fp32")}
+    f64 = (6.0 if forms["contract_fp64"] == "gauss" else 8.0) * n * chi_p * chi * d  # MFMA flops executed
+    f32 = (6.0 if forms["contract_fp32"] == "gauss" else 8.0) * n * chi_p * chi * d
     out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_fp64_s": t64, "contract_fp32_s": t32,
-           "sample_site_s": ts, "contract_fp64_tflops": flops / t64 / 1e12, "contract_fp32_tflops": flops / t32 / 1e12,
+           "sample_site_s": ts, "contract_forms": forms, "contract_fp64_tflops": f64 / t64 / 1e12,
+           "contract_fp32_tflops": f32 / t32 / 1e12,
            "chain_samples_per_s_fp64": n / ((inst.M - 1) * (t64 + ts)),
            "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts))}
     log(f"sampling probe at site {k}: {out}")
@@ -164,7 +167,7 @@ def partial_run(args):
     inst = instances.workload(args.workload)
     sites = [int(x) for x in args.partial_sites.split(",")]
     runtime.context()
-    autotune.load_or_tune()
+    tuning = autotune.load_or_tune()
     log("tuned")
     t0 = time.perf_counter()
     ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
@@ -197,7 +200,7 @@ def partial_run(args):
     supported = [w for w in work if w[2] <= _lib.MAX_HAFNIAN_SIZE]
     sampling_rate = None
     if sample_probe is not None:
-        sampling_rate = probe_sampling(inst, ko, sample_probe, args.partial_sample)
+        sampling_rate = probe_sampling(inst, ko, sample_probe, args.partial_sample, tuning)
     out = {"workload": args.workload, "partial": True, "M": inst.M, "d": inst.d, "chi": inst.chi,
            "sampling_probe": sampling_rate,
            "kron_s": t_kron, "sites": rows,
@@ -344,6 +347,8 @@ def main():
                          "extrapolate the whole construction (for instances too large for one run)")
     ap.add_argument("--partial-sample", type=int, default=0,
                     help="with --partial-sites including M//2: time that site's sampling step for this many samples")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="run the batched form over this many random instances (graph-captured sampling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -361,6 +366,18 @@ def main():
         args.no_cpu_baseline = True
     if args.partial_sites:
         partial_run(args)
+        return
+    if args.batched:
+        import torch
+        from bosonsupremacy_amd import autotune, batched, instances, runtime
+        runtime.context()
+        tuning = autotune.load_or_tune()
+        insts = instances.random_batch(args.batched, seed=1000, n_samples=args.samples or 10 ** 4)
+        out = batched.run_batch(insts, n_samples=args.samples or 10 ** 4, verbose=True)
+        out.update({"metric": "instances/s (construction + one graph-replayed sampling pass, FP64 or FP32 form)",
+                    "workload": f"{args.batched} random instances, M~U{{16..96}}, d~U{{4..8}}, chi~U{{128..1024}}",
+                    "tuning": {str(k): v for k, v in tuning.items()}})
+        print(json.dumps(out))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
@@ -414,7 +431,10 @@ def main():
     kron_out, G, L, samples = last
     n_haf, haf_flops = site_hafnian_work(kron_out[0], kron_out[1], inst.d)
     chi, d = inst.chi, inst.d
-    gemm_flops = 8.0 * n_samples * chi * chi * d      # per site_contract launch
+    # per site_contract launch: MFMA flops of the contraction form in use (2x2 real blocks: 8 per
+    # complex multiply-add; Gauss: three real products, 6)
+    form64 = tuning.get("contract_fp64", "blocks")
+    gemm_flops = (6.0 if form64 == "gauss" else 8.0) * n_samples * chi * chi * d
     kernels = {}
     for name, ds in kern.items():
         per_step = len(ds) / args.steps
@@ -436,7 +456,8 @@ def main():
         ach = kernels[dom]["tflops"]
         roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.workload, n_samples),
-                "algorithmic": f"8*n*chi^2*d = {gemm_flops:.3e} FP64 flops per launch",
+                "algorithmic": (f"{'6' if form64 == 'gauss' else '8'}*n*chi^2*d = {gemm_flops:.3e} FP64 MFMA "
+                                f"flops per launch (contraction form: {form64})"),
                 "algorithmic_bytes": 16.0 * (n_samples * chi + chi * chi * d + n_samples * chi * d)}
     elif dom == "gamma_site":
         ach = kernels[dom]["tflops"]

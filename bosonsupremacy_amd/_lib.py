@@ -13,7 +13,7 @@ BSMPS_OK = 0
 BSMPS_ERR_INVALID = 1
 BSMPS_ERR_HIP = 2
 BSMPS_ERR_UNSUPPORTED = 3
-MAX_HAFNIAN_SIZE = 26
+MAX_HAFNIAN_SIZE = 34
 
 SITE_FIRST, SITE_MIDDLE, SITE_LAST = 0, 1, 2
 
@@ -56,6 +56,7 @@ _PROTOTYPES = {
     "bsmps_set_group_lanes": ([c_void_p, c_int, c_int], c_int),
     "bsmps_set_split": ([c_void_p, c_int, c_int], c_int),
     "bsmps_set_split_budget": ([c_void_p, c_size_t], c_int),
+    "bsmps_set_contract_form": ([c_void_p, c_int, c_int], c_int),
     "bsmps_sigma_select": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_dense": ([c_void_p, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p], c_int),

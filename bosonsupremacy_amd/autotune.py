@@ -1,13 +1,16 @@
 """Launch-shape autotuner for the hafnian kernels.
 
 For every even hafnian size it times the candidate launch shapes -- one lane per matrix
-(sizes <= 10), 16 / 32 / 64 cooperating lanes per matrix, and for sizes 12..26 the
+(sizes <= 10), 16 / 32 / 64 cooperating lanes per matrix, and for sizes 12..34 the
 frontier-split form (upper recursion tree breadth-first in HBM, one lane per subtree below)
--- on synthetic targets drawn from a random Sigma, and keeps the fastest.
+-- on synthetic targets drawn from a random Sigma, and keeps the fastest.  It also times the
+two forms of the site contraction (complex product as one real GEMM of 2x2 blocks, or as
+three real products) for FP64 and FP32 at a sampling-sized shape and keeps the faster.
 Every candidate computes bit-identical results (the tests check all shapes against the same
 oracle), so the choice only affects speed.  Results are cached per device name in
 ``~/.cache/bsmps/tuning.json`` (or $BSMPS_TUNING_CACHE) and applied to the process-wide
-context.  A choice is "lane", "group:G" or "split".
+context.  A hafnian choice is "lane", "group:G" or "split"; a contraction choice
+("contract_fp64", "contract_fp32") is "blocks" or "gauss".
 """
 import json
 import os
@@ -19,8 +22,9 @@ from . import runtime
 
 GROUP_CANDIDATES = (16, 32, 64)
 LANE_MAX_SIZE = 10
-SPLIT_SIZES = range(12, 27, 2)
-CACHE_VERSION = "v4"
+SPLIT_SIZES = range(12, 35, 2)
+CACHE_VERSION = "v5"
+CONTRACT_KEYS = {"contract_fp64": 0, "contract_fp32": 1}
 
 
 def _cache_path():
@@ -78,9 +82,43 @@ def tune(sizes=(6, 8, 10, 12, 14, 16, 18), ld=64, items=None, seed=0, verbose=Fa
     return best
 
 
-def apply(best):
-    """Install a tuning {n: "lane" | "group:G" | "split"} on the process-wide context."""
+def tune_contract(n=16384, chi=512, d=10, verbose=False):
+    """Time both site-contraction forms per precision on a [n, chi] x [chi, chi, d] product;
+    returns {"contract_fp64": form, "contract_fp32": form}."""
     ctx = runtime.context()
+    dev = ctx.device
+    gen = torch.Generator(device=dev).manual_seed(0)
+    best = {}
+    for key, prec in CONTRACT_KEYS.items():
+        dt = torch.complex128 if prec == 0 else torch.complex64
+        entry = "bsmps_site_contract" if prec == 0 else "bsmps_site_contract_c64"
+        pre = torch.randn((n, chi), dtype=dt, device=dev, generator=gen)
+        lam = torch.rand((chi,), dtype=torch.float64, device=dev, generator=gen)
+        G = torch.randn((chi, chi, d), dtype=dt, device=dev, generator=gen)
+        T = torch.empty((n, chi, d), dtype=dt, device=dev)
+
+        def run():
+            ctx.call(entry, n, chi, chi, d, runtime.ptr(pre), runtime.ptr(lam), runtime.ptr(G), runtime.ptr(T))
+        timings = {}
+        for form in ("blocks", "gauss"):
+            ctx.call("bsmps_set_contract_form", prec, int(form == "gauss"))
+            timings[form] = _time_call(run)
+        best[key] = min(timings, key=timings.get)
+        ctx.call("bsmps_set_contract_form", prec, int(best[key] == "gauss"))
+        if verbose:
+            print(f"{key}: " + " ".join(f"{k}:{v:.2f}ms" for k, v in timings.items()), flush=True)
+        del pre, lam, G, T
+    return best
+
+
+def apply(best):
+    """Install a tuning {n: "lane" | "group:G" | "split", "contract_*": "blocks" | "gauss"} on
+    the process-wide context."""
+    ctx = runtime.context()
+    for key, prec in CONTRACT_KEYS.items():
+        if key in best:
+            ctx.call("bsmps_set_contract_form", prec, int(best[key] == "gauss"))
+    best = {n: v for n, v in best.items() if not isinstance(n, str)}
     lane_sizes = [n for n, v in best.items() if v == "lane"]
     lane_max_m = 2
     for n in sorted(lane_sizes):
@@ -111,6 +149,7 @@ def load_or_tune(verbose=False):
     entry = cache.get(f"{name}/{CACHE_VERSION}")
     if entry is None:
         best = tune(verbose=verbose)
+        best.update(tune_contract(verbose=verbose))
         entry = {str(k): v for k, v in best.items()}
         cache[f"{name}/{CACHE_VERSION}"] = entry
         try:
@@ -119,7 +158,7 @@ def load_or_tune(verbose=False):
                 json.dump(cache, f, indent=1)
         except OSError:
             pass
-    best = {int(k): v for k, v in entry.items()}
+    best = {int(k) if k.isdigit() else k: v for k, v in entry.items()}
     apply(best)
     return best
 

@@ -55,14 +55,15 @@ struct bsmps_ctx {
     size_t ws_bytes;
     // autotuned launch parameters
     int lane_max_m;   // hafnians of size <= 2*lane_max_m take the one-lane-per-hafnian path
-    int group_lanes_by_m[16];  // lanes per hafnian in the cooperative path (16, 32 or 64), by n/2
+    int group_lanes_by_m[BSMPS_MAX_HAFNIAN_SIZE / 2 + 1];  // lanes per hafnian in the cooperative path (16, 32 or 64), by n/2
     int max_lds;      // bytes of LDS one workgroup may use on this device
-    int split_by_m[16];  // 1: hafnians of size 2m take the frontier-split path (n = 12..26)
+    int split_by_m[BSMPS_MAX_HAFNIAN_SIZE / 2 + 1];  // 1: size 2m takes the frontier-split path (n = 12..34)
     // device scratch of the frontier-split hafnian path (separate from `ws`, which holds the
     // per-call tables that are live while the hafnian launches run)
     void* hws;
     size_t hws_bytes;
     size_t split_chunk_bytes;  // upper bound of `hws` per chunk of hafnians
+    int contract_gauss[2];     // site contraction form per precision (0 FP64, 1 FP32): 1 = three real products
     // two pinned host staging buffers for small per-call tables, used alternately; an event
     // marks when the copy out of each has completed
     void* pin[2];

@@ -94,11 +94,13 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     c->ws = nullptr;
     c->ws_bytes = 0;
     c->lane_max_m = 4;
-    for (int m = 0; m < 16; ++m) c->group_lanes_by_m[m] = 64;
-    for (int m = 0; m < 16; ++m) c->split_by_m[m] = (m >= 6 && m <= 13) ? 1 : 0;
+    for (int m = 0; m <= BSMPS_MAX_HAFNIAN_SIZE / 2; ++m) c->group_lanes_by_m[m] = 64;
+    for (int m = 0; m <= BSMPS_MAX_HAFNIAN_SIZE / 2; ++m) c->split_by_m[m] = m >= 6 ? 1 : 0;
     c->hws = nullptr;
     c->hws_bytes = 0;
     c->split_chunk_bytes = (size_t)4 << 30;
+    c->contract_gauss[0] = 0;
+    c->contract_gauss[1] = 0;
     for (int i = 0; i < 2; ++i) {
         c->pin[i] = nullptr;
         c->pin_bytes[i] = 0;
@@ -150,7 +152,7 @@ extern "C" int bsmps_set_tuning(bsmps_ctx* ctx, int lane_max_m, int group_lanes)
         return BSMPS_ERR_INVALID;
     }
     ctx->lane_max_m = lane_max_m;
-    for (int m = 0; m < 16; ++m) ctx->group_lanes_by_m[m] = group_lanes;
+    for (int m = 0; m <= BSMPS_MAX_HAFNIAN_SIZE / 2; ++m) ctx->group_lanes_by_m[m] = group_lanes;
     return BSMPS_OK;
 }
 
@@ -166,11 +168,20 @@ extern "C" int bsmps_set_group_lanes(bsmps_ctx* ctx, int n, int group_lanes) {
 }
 
 extern "C" int bsmps_set_split(bsmps_ctx* ctx, int n, int enable) {
-    if (!ctx || n < 12 || n > 26 || n % 2 || (enable != 0 && enable != 1)) {
-        bsmps_set_error("bsmps_set_split: n must be even in 12..26 and enable 0 or 1");
+    if (!ctx || n < 12 || n > BSMPS_MAX_HAFNIAN_SIZE || n % 2 || (enable != 0 && enable != 1)) {
+        bsmps_set_error("bsmps_set_split: n must be even in 12..%d and enable 0 or 1", BSMPS_MAX_HAFNIAN_SIZE);
         return BSMPS_ERR_INVALID;
     }
     ctx->split_by_m[n / 2] = enable;
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_set_contract_form(bsmps_ctx* ctx, int precision, int gauss) {
+    if (!ctx || precision < 0 || precision > 1 || (gauss != 0 && gauss != 1)) {
+        bsmps_set_error("bsmps_set_contract_form: precision must be 0 (FP64) or 1 (FP32), gauss 0 or 1");
+        return BSMPS_ERR_INVALID;
+    }
+    ctx->contract_gauss[precision] = gauss;
     return BSMPS_OK;
 }
 

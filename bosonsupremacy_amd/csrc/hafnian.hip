@@ -14,7 +14,7 @@
 //   * group path  (n >= 6):  G = 16/32/64 lanes per matrix follow a precomputed depth-first
 //                 schedule of the recursion tree; each node's child matrix is built with one
 //                 lane per entry, the polynomial stack lives in LDS.
-//   * split path  (n = 12..26): the upper recursion tree breadth-first in HBM, one launch
+//   * split path  (n = 12..34): the upper recursion tree breadth-first in HBM, one launch
 //                 per level and one lane per (matrix, child entry); then one lane per
 //                 (matrix, subtree) for every subtree that fits one lane's registers; then a
 //                 tree-order combine.
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Frontier split (sizes 12..26).  The recursion tree is cut at "frontier" nodes whose subtree
+// Frontier split (sizes 12..34).  The recursion tree is cut at "frontier" nodes whose subtree
 // fits one lane's registers.  The part above the frontier runs breadth-first, one launch per
 // tree level, with one lane per (hafnian, child-matrix entry): every node that owns new
 // polynomials (the root and each right child) keeps them in HBM, laid out
@@ -708,13 +708,14 @@ __global__ __launch_bounds__(kWave) void haf_bfs_level_kernel(BfsData bd, const 
     if (h >= bd.n_h) return;
     const int2 it = items[blockIdx.x];
     const BfsOp op = ops[it.x];
-    // the plan only makes degrees 0, 1, 3, 7 and M (checked on the host); M first, since it
-    // can coincide with 7
+    // the plan only makes degrees 0, 1, 3, 7, 15 and M (checked on the host); M first, since
+    // it can coincide with 7 or 15
     if (op.D == M) bfs_item<M, M, M>(bd, op, it.y, h);
     else if (op.D == 0) bfs_item<M, 0, cmin(M, 1)>(bd, op, it.y, h);
     else if (op.D == 1) bfs_item<M, 1, cmin(M, 3)>(bd, op, it.y, h);
     else if (op.D == 3) { if constexpr (M > 3) bfs_item<M, 3, cmin(M, 7)>(bd, op, it.y, h); }
     else if (op.D == 7) { if constexpr (M > 7) bfs_item<M, 7, cmin(M, 15)>(bd, op, it.y, h); }
+    else if (op.D == 15) { if constexpr (M > 15) bfs_item<M, 15, cmin(M, 31)>(bd, op, it.y, h); }
 }
 
 template <int OFF>
@@ -897,12 +898,13 @@ struct BfsBuilder {
 };
 
 // Subtree shapes with a compiled one-lane kernel: (K0, D) = (M-4, 0), (M-4, 1), (M-3, 3),
-// (M-2, 7) and (M-2, M), for M = 6..13.
-constexpr int kSplitMinM = 6, kSplitMaxM = 13;
+// (M-2, 7), (M-2, 15) and (M-2, M), for M = 6..17 (a node's degree is 2^r - 1 after r RIGHT
+// steps, capped at M, so up to M = 17 every degree is 0, 1, 3, 7, 15 or M).
+constexpr int kSplitMinM = 6, kSplitMaxM = BSMPS_MAX_HAFNIAN_SIZE / 2;
 bool subtree_compiled(int M, int K0, int D) {
     if (M < kSplitMinM || M > kSplitMaxM) return false;
     return (K0 == M - 4 && (D == 0 || D == 1)) || (K0 == M - 3 && D == 3) ||
-           (K0 == M - 2 && (D == M || (D == 7 && M > 7)));
+           (K0 == M - 2 && (D == M || (D == 7 && M > 7) || (D == 15 && M > 15)));
 }
 
 struct FrontierPlan {
@@ -954,7 +956,7 @@ int get_fplan(int device, int m, FrontierPlan** out) {
             }
         }
         for (const BfsOp& op : fb.ops)
-            if (!(op.D == 0 || op.D == 1 || op.D == 3 || op.D == 7 || op.D == m)) p.ok = false;
+            if (!(op.D == 0 || op.D == 1 || op.D == 3 || op.D == 7 || op.D == 15 || op.D == m)) p.ok = false;
         p.n_front = (int)fb.front.size();
         std::map<std::pair<int, int>, std::vector<int>> by_type;
         for (int f = 0; f < p.n_front; ++f) {
@@ -1004,6 +1006,7 @@ int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const BfsData& b
     else if (K0 == M - 3 && D == 3) BSMPS_SUBTREE(M - 3, 3);
     else if (K0 == M - 2 && D == M) BSMPS_SUBTREE(M - 2, M);
     else if (K0 == M - 2 && D == 7 && M > 7) BSMPS_SUBTREE(M - 2, (M > 7 ? 7 : M));
+    else if (K0 == M - 2 && D == 15 && M > 15) BSMPS_SUBTREE(M - 2, (M > 15 ? 15 : M));
     else {
         bsmps_set_error("no subtree kernel for size %d, level %d, degree %d", 2 * M, K0, D);
         return BSMPS_ERR_UNSUPPORTED;
@@ -1053,9 +1056,13 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
             BSMPS_SPLIT_CASE(11)
             BSMPS_SPLIT_CASE(12)
             BSMPS_SPLIT_CASE(13)
+            BSMPS_SPLIT_CASE(14)
+            BSMPS_SPLIT_CASE(15)
+            BSMPS_SPLIT_CASE(16)
+            BSMPS_SPLIT_CASE(17)
 #undef BSMPS_SPLIT_CASE
             default:
-                bsmps_set_error("frontier-split hafnian path supports sizes 12..26, got %d", 2 * m);
+                bsmps_set_error("frontier-split hafnian path supports sizes 12..%d, got %d", BSMPS_MAX_HAFNIAN_SIZE, 2 * m);
                 return BSMPS_ERR_INVALID;
         }
         hipLaunchKernelGGL((haf_combine_kernel<S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, nh, H, vals,

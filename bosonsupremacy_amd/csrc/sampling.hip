@@ -233,6 +233,141 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
 }
 
 // ---------------------------------------------------------------------------------------
+// Site contraction, Gauss form: T[s][c] = sum_b (pre[s][b] lambda[b]) Gamma[b][c], complex,
+// stored interleaved (T's layout).  The complex product runs as three real products
+// (Gauss's method: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
+// Im = P3 - P1 - P2), 6 n chi_prev chi d real MFMA flops instead of the 8 of the complex
+// product; the extra rounding is a few ulps of |A||B| (the FP64 form stays within 1e-13 of
+// the exact product).  Workgroup tile 128 rows x 64 complex columns x 8 complex k (256
+// threads, four waves of 64 rows x 32 complex columns, 3 x 4 x 2 MFMA 16x16 blocks each); the
+// next K tile is fetched into registers while the current one computes.
+// Selected per precision by bsmps_set_contract_form (the autotuner times both forms).
+// ---------------------------------------------------------------------------------------
+template <typename E>
+__global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
+                                                        const E* __restrict__ pre, const double* __restrict__ lam,
+                                                        const E* __restrict__ gamma, E* __restrict__ T) {
+    typedef MfmaTraits<E> Tr;
+    typedef typename Tr::acc_t acc_t;
+    typedef typename Tr::pair_t E2;
+    constexpr int KC = TK / 2;   // complex k per tile
+    constexpr int NC = TN / 2;   // complex columns per tile
+    __shared__ E Ar[KC][TM + 2], Ai[KC][TM + 2];  // [k][row]
+    __shared__ E Br[KC][NC + 2], Bi[KC][NC + 2];  // [k][complex col]
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int64_t row0 = (int64_t)blockIdx.y * TM;
+    const int64_t ccol0 = (int64_t)blockIdx.x * NC;  // complex column
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * (NC / 2);
+
+    // three real products per 16x16 block (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)
+    acc_t p1[4][2], p2[4][2], p3[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = acc_t{0, 0, 0, 0};
+
+    // per thread: 4 complex of the A tile (128 rows x 8 b) and 2 of the B tile (8 b x 64 c)
+    E2 ra[4], rb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / KC, bb = e % KC;
+            const int64_t srow = row0 + r;
+            const int b = k0 + bb;
+            E2 v;
+            v.x = 0;
+            v.y = 0;
+            if (srow < n && b < chi_prev) {
+                v = reinterpret_cast<const E2*>(pre)[srow * chi_prev + b];
+                const double l = lam[b];
+                v.x = (E)(v.x * l);
+                v.y = (E)(v.y * l);
+            }
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / NC, cc = e % NC;
+            const int b = k0 + bb;
+            const int64_t c = ccol0 + cc;
+            E2 g;
+            g.x = 0;
+            g.y = 0;
+            if (b < chi_prev && c < ncols) g = reinterpret_cast<const E2*>(gamma)[(int64_t)b * ncols + c];
+            rb[q] = g;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / KC, bb = e % KC;
+            Ar[bb][r] = ra[q].x;
+            Ai[bb][r] = ra[q].y;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = tid + 256 * q;
+            const int bb = e / NC, cc = e % NC;
+            Br[bb][cc] = rb[q].x;
+            Bi[bb][cc] = rb[q].y;
+        }
+    };
+
+    fetch(0);
+    for (int k0 = 0; k0 < chi_prev; k0 += KC) {
+        stash();
+        __syncthreads();
+        if (k0 + KC < chi_prev) fetch(k0 + KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            // operand layout: lane holds A[lane%16][lane/16] and B[lane/16][lane%16]
+            E ar[4], ai[4], as[4], br[2], bi[2], bs[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar[i] = Ar[kk + lane / 16][wm + i * 16 + lane % 16];
+                ai[i] = Ai[kk + lane / 16][wm + i * 16 + lane % 16];
+                as[i] = ar[i] + ai[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                br[j] = Br[kk + lane / 16][wn + j * 16 + lane % 16];
+                bi[j] = Bi[kk + lane / 16][wn + j * 16 + lane % 16];
+                bs[j] = br[j] + bi[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    p1[i][j] = Tr::mfma(ar[i], br[j], p1[i][j]);
+                    p2[i][j] = Tr::mfma(ai[i], bi[j], p2[i][j]);
+                    p3[i][j] = Tr::mfma(as[i], bs[j], p3[i][j]);
+                }
+        }
+        __syncthreads();
+    }
+    // re = P1 - P2, im = (P3 - P1) - P2
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t srow = row0 + wm + i * 16 + Tr::row(lane, r);
+                const int64_t c = ccol0 + wn + j * 16 + lane % 16;
+                if (srow < n && c < ncols) {
+                    E2 v;
+                    v.x = p1[i][j][r] - p2[i][j][r];
+                    v.y = (p3[i][j][r] - p1[i][j][r]) - p2[i][j][r];
+                    reinterpret_cast<E2*>(T)[srow * ncols + c] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------------------
 // BF16x3 split-precision form of the contraction (v_mfma_f32_16x16x32_bf16, FP32
 // accumulation).  Every FP32 operand x is split as x = hi + lo with hi = bf16(x) and
 // lo = bf16(x - hi); a product is taken as hi*hi + hi*lo + lo*hi (the lo*lo term, ~2^-16
@@ -638,12 +773,16 @@ extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int 
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+    if (gx > 0x7fffffff || gy > 65535) {
         bsmps_set_error("bsmps_site_contract: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
-                       chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    if (ctx->contract_gauss[0])
+        hipLaunchKernelGGL(site_gemm3m_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    else
+        hipLaunchKernelGGL(site_gemm_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
@@ -705,12 +844,16 @@ extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, 
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+    if (gx > 0x7fffffff || gy > 65535) {
         bsmps_set_error("bsmps_site_contract_c64: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
-                       chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    if (ctx->contract_gauss[1])
+        hipLaunchKernelGGL(site_gemm3m_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    else
+        hipLaunchKernelGGL(site_gemm_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
@@ -772,7 +915,7 @@ extern "C" int bsmps_site_contract_bf16x3(bsmps_ctx* ctx, int64_t n, int chi_pre
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    if (gx > 65535 * 16 || gy > 0x7fffffff) {
+    if (gx > 0x7fffffff || gy > 65535) {
         bsmps_set_error("bsmps_site_contract_bf16x3: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }

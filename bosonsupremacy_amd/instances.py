@@ -104,3 +104,18 @@ def workload(name, seed=0, **overrides):
     params = dict(WORKLOADS[name])
     params.update(overrides)
     return make_instance(name, seed=seed, **params)
+
+
+def random_batch(n_instances=256, seed=1000, n_samples=10 ** 4):
+    """The batched workload: independent instances with M ~ U{16..96}, d ~ U{4..8},
+    chi ~ U{128..1024}, r_i ~ U(0.5, 1.5), eta ~ U(0.3, 0.7), instance i with its own Haar
+    unitary (seed + i); dd = 2 d (the photon cutoff after displacement)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_instances):
+        M = int(rng.integers(16, 97))
+        d = int(rng.integers(4, 9))
+        chi = int(rng.integers(128, 1025))
+        eta = float(rng.uniform(0.3, 0.7))
+        out.append(make_instance(f"batch_{i}", M, 0.5, 1.5, eta, d, chi, 2 * d, n_samples, seed=seed + i))
+    return out

@@ -35,7 +35,8 @@ def _require_gpu():
 class Context:
     """Owns a bsmps_ctx bound to ``device`` and enqueuing on the torch stream current at
     creation (normally torch's default stream, HIP's null stream), so library kernels are
-    ordered with torch's allocations, fills, frees and copies of the same buffers."""
+    ordered with torch's allocations, fills, frees and copies of the same buffers.  A context
+    created inside ``torch.cuda.stream(s)`` enqueues on s (used for graph capture)."""
 
     def __init__(self, device=0):
         from . import hostpool
@@ -48,9 +49,25 @@ class Context:
         _lib.check(lib.bsmps_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(handle)))
         self.handle = handle
         self.lib = lib
+        self.settings = {}
+
+    # launch-tuning setters; their last successful calls are kept so another context can
+    # be given the same settings (copy_settings)
+    _SETTERS = {"bsmps_set_tuning": 0, "bsmps_set_group_lanes": 1, "bsmps_set_split": 1,
+                "bsmps_set_split_budget": 0, "bsmps_set_contract_form": 1}
 
     def call(self, name, *args):
-        return _lib.check(getattr(self.lib, name)(self.handle, *args))
+        status = _lib.check(getattr(self.lib, name)(self.handle, *args))
+        if name in self._SETTERS:
+            if name == "bsmps_set_tuning":  # resets every size's lane count
+                self.settings = {k: v for k, v in self.settings.items() if k[0] != "bsmps_set_group_lanes"}
+            self.settings[(name,) + tuple(args[:self._SETTERS[name]])] = args
+        return status
+
+    def copy_settings(self, other):
+        """Apply ``other``'s launch-tuning settings to this context, in the order they were made."""
+        for key, args in list(other.settings.items()):
+            self.call(key[0], *args)
 
     def synchronize(self):
         self.call("bsmps_synchronize")

@@ -101,8 +101,8 @@ class Sampler:
 
     PRECISIONS = ("fp64", "fp32", "bf16x3")
 
-    def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64"):
-        self.ctx = runtime.context()
+    def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64", ctx=None):
+        self.ctx = ctx or runtime.context()
         dev = self.ctx.device
         self.M = len(gammas)
         self.chis = [int(g.shape[1]) for g in gammas]

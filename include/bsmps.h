@@ -32,7 +32,7 @@ enum {
 };
 
 /* Largest hafnian matrix size (rows) the hafnian entry points accept. */
-#define BSMPS_MAX_HAFNIAN_SIZE 26
+#define BSMPS_MAX_HAFNIAN_SIZE 34
 
 typedef struct bsmps_ctx bsmps_ctx;
 
@@ -54,15 +54,23 @@ int bsmps_version(void);
  * for every size (bsmps_set_tuning) or for size n only (bsmps_set_group_lanes). */
 int bsmps_set_tuning(bsmps_ctx *ctx, int lane_max_m, int group_lanes);
 int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
-/* Hafnians of size n in 12..26 take the frontier-split path when `enable` is 1 (default): the
+/* Hafnians of size n in 12..34 take the frontier-split path when `enable` is 1 (default): the
  * upper part of the recursion tree runs breadth-first, one launch per tree level, with its
  * polynomials in device scratch, and every subtree small enough for one lane's registers is
  * evaluated by a one-lane-per-subtree kernel; results are identical to the unsplit path (no
  * reference counterpart). */
 int bsmps_set_split(bsmps_ctx *ctx, int n, int enable);
 /* Device scratch the split path may hold (default 4 GiB, at least 1 MiB): batches of
- * hafnians are processed in chunks whose frontier data fits it. */
+ * hafnians are processed in chunks whose frontier data fits it, with at least 64 hafnians
+ * per chunk (one wavefront), which alone needs 2.2 GB at size 30, 4.8 GB at 32 and 10.3 GB
+ * at 34. */
 int bsmps_set_split_budget(bsmps_ctx *ctx, size_t bytes);
+/* Form of the site contraction for `precision` 0 (bsmps_site_contract, FP64) or 1
+ * (bsmps_site_contract_c64, FP32): gauss = 0 runs the complex product as one real GEMM with
+ * 2x2 real blocks (8 n chi_prev chi d flops, default), gauss = 1 as three real products
+ * (Gauss: 6 n chi_prev chi d flops, more registers per tile, a few ulps more rounding).
+ * Launch tuning only; no reference counterpart. */
+int bsmps_set_contract_form(bsmps_ctx *ctx, int precision, int gauss);
 
 /* ---- hafnians (L3) --------------------------------------------------------------------- */
 

@@ -39,5 +39,6 @@ def test_error_paths_without_device():
     assert lib.bsmps_create(0, None, None) == _lib.BSMPS_ERR_INVALID
     assert lib.bsmps_set_split(None, 12, 1) == _lib.BSMPS_ERR_INVALID
     assert lib.bsmps_set_split_budget(None, 1 << 30) == _lib.BSMPS_ERR_INVALID
+    assert lib.bsmps_set_contract_form(None, 0, 1) == _lib.BSMPS_ERR_INVALID
     with pytest.raises(_lib.BsmpsError):
         _lib.check(lib.bsmps_synchronize(None))

@@ -1,0 +1,58 @@
+"""Batched form: the graph-captured sampling pass equals the eager pass on every replay, and a
+small batch of random instances runs end to end."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("gauss", [0, 1], ids=["blocks", "gauss"])
+def test_graph_sampler_replays_match_eager(gauss):
+    """Every replay equals the eager pass; the capture context carries the process-wide
+    contraction form (both forms)."""
+    from bosonsupremacy_amd import batched, instances, mps, runtime, sampling
+    ctx = runtime.context()
+    for prec in (0, 1):
+        ctx.call("bsmps_set_contract_form", prec, gauss)
+    try:
+        _replays_match(batched, instances, mps, sampling)
+    finally:
+        for prec in (0, 1):
+            ctx.call("bsmps_set_contract_form", prec, 0)
+
+
+def _replays_match(batched, instances, mps, sampling):
+    inst = instances.workload("m8_d4_chi64")
+    G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 700
+    for prec in ("fp64", "fp32"):
+        eager = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
+        gs = batched.GraphSampler(G, L, sqrtW, inst.dd, n, prec, return_probs=True)
+        for rep in range(3):
+            normals, uniforms = sampling.random_inputs(inst.M, n, seed=11, offset=rep * n)
+            want_s, want_p = eager.run(normals, uniforms, return_probs=True)
+            torch.cuda.synchronize()
+            got_s, got_p = gs.run(normals, uniforms)
+            torch.cuda.synchronize()
+            print(f"{prec} replay {rep}", flush=True)
+            assert torch.equal(got_s, want_s), (prec, rep)
+            assert torch.equal(got_p, want_p), (prec, rep)
+            # eager work between replays
+            eager.run(*sampling.random_inputs(inst.M, n, seed=12, offset=rep))
+            torch.cuda.synchronize()
+
+
+def test_small_random_batch_runs():
+    from bosonsupremacy_amd import batched, instances
+    insts = instances.random_batch(3, seed=7, n_samples=500)
+    for inst in insts:  # keep the test small
+        inst.chi = min(inst.chi, 128)
+        inst.M = inst.M
+    out = batched.run_batch(insts, n_samples=500)
+    print({k: v for k, v in out.items() if k != "per_instance"}, flush=True)
+    assert out["instances"] + len(out["skipped"]) == 3 and out["instances"] >= 1
+    assert out["fp32_marginal_abs_err_max"] < 1e-6
+    assert out["fp32_off_fp64_path_total"] == 0
+    assert out["instances_per_s_fp64"] > 0 and out["instances_per_s_fp32"] > 0

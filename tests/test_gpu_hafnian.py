@@ -17,7 +17,7 @@ def ctx():
     c = runtime.context()
     yield c
     c.set_tuning(4, 64)
-    for n in range(12, 27, 2):
+    for n in range(12, 31, 2):
         c.call("bsmps_set_split", n, 1)
     c.call("bsmps_set_split_budget", 4 << 30)
 
@@ -43,7 +43,7 @@ def test_targets_match_reference_bits(ctx, golden_m8, tuning):
         assert same_bits(vals, golden_m8[f"site_haf_{n}"]), n
 
 
-@pytest.mark.parametrize("n", [16, 18, 20, 22, 24, 26])
+@pytest.mark.parametrize("n", [16, 18, 20, 22, 24, 26, 28, 30, 32, 34])
 def test_large_sizes_match_c_oracle(ctx, n):
     from bosonsupremacy_amd import mps
     ctx.set_tuning(4, 64)
@@ -77,7 +77,7 @@ def test_many_items_all_paths_agree(ctx):
             assert same_bits(o, outs[0])
 
 
-@pytest.mark.parametrize("n", [12, 14, 16, 18, 20, 22, 24, 26])
+@pytest.mark.parametrize("n", [12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34])
 def test_split_matches_unsplit_and_oracle(ctx, n):
     """The frontier-split path (upper tree on cooperating lanes, subtrees one per lane) gives
     the same bits as the unsplit cooperative path and the C oracle, for every lane count and
@@ -87,10 +87,10 @@ def test_split_matches_unsplit_and_oracle(ctx, n):
     rng = np.random.default_rng(100 + n)
     ld = 30
     S = (rng.normal(size=(ld, ld)) + 1j * rng.normal(size=(ld, ld))) * 0.4
-    count = {12: 3000, 14: 1500, 16: 700, 18: 300, 20: 130, 22: 70, 24: 40, 26: 20}[n]
+    count = {12: 3000, 14: 1500, 16: 700, 18: 300, 20: 130, 22: 70, 24: 40, 26: 20, 28: 10, 30: 6, 32: 4, 34: 3}[n]
     tg = rng.integers(0, ld, size=(count, n)).astype(np.int32)
     den = rng.uniform(0.5, 2.0, size=count)
-    k = 12
+    k = min(12, count)
     want = c_oracle.hafnian_targets(S, tg[:k]) / den[:k]
     outs = {}
     for split in (0, 1):
@@ -133,10 +133,10 @@ def test_sigma_select_and_odd_sizes(ctx):
 def test_invalid_sizes_raise(ctx):
     from bosonsupremacy_amd import _lib, mps
     with pytest.raises(_lib.BsmpsError):
-        mps.A_elem(np.eye(4), np.zeros((2, 28), np.int32), np.ones(2))
+        mps.A_elem(np.eye(4), np.zeros((2, 36), np.int32), np.ones(2))
     with pytest.raises(_lib.BsmpsError):
         ctx.set_tuning(6, 64)
     with pytest.raises(_lib.BsmpsError):
-        ctx.call("bsmps_set_split", 28, 1)
+        ctx.call("bsmps_set_split", 36, 1)
     with pytest.raises(_lib.BsmpsError):
         ctx.call("bsmps_set_split_budget", 1000)

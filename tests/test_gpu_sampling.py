@@ -20,11 +20,23 @@ def test_batch_displaces_matches_expm():
         assert np.abs(got - want).max() < 1e-12, dd
 
 
-def test_site_contract_matches_matmul():
+@pytest.fixture(params=[0, 1], ids=["blocks", "gauss"])
+def contract_form(request):
+    """Both site-contraction forms (real 2x2 blocks, Gauss three products) for FP64 and FP32."""
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    for prec in (0, 1):
+        ctx.call("bsmps_set_contract_form", prec, request.param)
+    yield request.param
+    for prec in (0, 1):
+        ctx.call("bsmps_set_contract_form", prec, 0)
+
+
+def test_site_contract_matches_matmul(contract_form):
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(1)
-    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4), (5, 100, 7, 6)):
+    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4), (5, 100, 7, 6), (300, 130, 70, 5)):
         pre = rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))
         lam = rng.uniform(0.1, 1, size=chi_p)
         G = rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))
@@ -55,7 +67,7 @@ def _mps_m8():
     return inst, G, L
 
 
-def test_sampling_m8_matches_oracle():
+def test_sampling_m8_matches_oracle(contract_form):
     from bosonsupremacy_amd import runtime, sampling
     inst, G, L = _mps_m8()
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
@@ -104,11 +116,11 @@ def test_sampling_m32_first_order_marginals():
     assert smp.min() >= 0 and smp.max() < inst.dd
 
 
-def test_site_contract_c64_matches_matmul():
+def test_site_contract_c64_matches_matmul(contract_form):
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(7)
-    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4)):
+    for n, chi_p, chi, d in ((70, 33, 17, 3), (256, 64, 64, 4), (300, 130, 70, 5)):
         pre = (rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))).astype(np.complex64)
         lam = rng.uniform(0.1, 1, size=chi_p)
         G = (rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))).astype(np.complex64)
@@ -121,7 +133,7 @@ def test_site_contract_c64_matches_matmul():
         assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max(), (n, chi_p, chi, d)
 
 
-def test_sampling_fp32_form_m8():
+def test_sampling_fp32_form_m8(contract_form):
     """FP32 contraction form: conditional distributions within 1e-5 of the FP64 form on the
     same random numbers, and the same draws away from cdf boundaries."""
     from bosonsupremacy_amd import runtime, sampling
