@@ -26,6 +26,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix (dense)
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 
 
 def hafnian_flops(m):
@@ -475,7 +477,7 @@ def main():
     from bosonsupremacy_amd import metrics
     pairs = [(i, i + 1) for i in range(inst.M - 1)] + [(0, inst.M - 1), (inst.M // 4, 3 * inst.M // 4)]
     ref = {}
-    for prec in (() if args.no_forms else ("fp64", "fp32", "bf16x3")):
+    for prec in (() if args.no_forms else ("fp64", "fp32", "bf16x3", "bf16x6")):
         s = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
         s.run(normals, uniforms)
         torch.cuda.synchronize()
@@ -483,10 +485,22 @@ def main():
         s.run(normals, uniforms)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        ft = KernelTimer()
+        ft.enabled = True
+        s.run(normals, uniforms, timer=ft)
+        t_contract = sum(ft.summary().get("site_contract", [0.0]))
         smp, probs = s.run(normals, uniforms, return_probs=True)
         rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
         e2 = metrics.rb_second_order(smp, probs)
-        forms[prec] = {"samples_per_s": n_samples / dt, "lambda_spectrum_rel_err_vs_fp64": 0.0}
+        # complex multiply-adds of all site contractions; MFMA flops executed per one of them:
+        # 8 (FP64 / FP32, 2x2 real blocks), 3 x 8 (BF16x3), 6 x 8 (BF16x6)
+        cmacs = sum(n_samples * s.chis[i - 1] * s.chis[i] * s.ds[i] for i in range(1, inst.M))
+        per, peak = {"fp64": (8, PEAK_FP64_TFLOPS), "fp32": (8, PEAK_FP32_MFMA_TFLOPS),
+                     "bf16x3": (24, PEAK_BF16_MFMA_TFLOPS), "bf16x6": (48, PEAK_BF16_MFMA_TFLOPS)}[prec]
+        forms[prec] = {"samples_per_s": n_samples / dt, "lambda_spectrum_rel_err_vs_fp64": 0.0,
+                       "contract_ms_per_pass": t_contract * 1e3,
+                       "contract_complex_gemm_tflops": 8 * cmacs / t_contract / 1e12,
+                       "contract_mfma_utilisation": per * cmacs / t_contract / 1e12 / peak}
         log(f"sampling form {prec}: {n_samples / dt:.0f} samples/s")
         if prec == "fp64":
             ref = {"e2": e2, "samples": smp}
@@ -509,7 +523,8 @@ def main():
         del probs, smp
 
     # learned-cutoff form (per-site cutoffs and ranks from the MLP, each checked exactly) with
-    # the FP32 contraction, against the plain FP64 form on the same random numbers
+    # the faster of the two FP32-accurate contractions (FP32, BF16x6), against the plain FP64
+    # form on the same random numbers
     learned_form = None
     if not args.no_learned:
         from bosonsupremacy_amd import learned
@@ -528,7 +543,9 @@ def main():
 
         learned_construction()
         t_lc, G2, L2, rep = learned_construction()
-        s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision="fp32")
+        accurate = [p for p in ("fp32", "bf16x6") if p in forms]
+        lprec = max(accurate, key=lambda p: forms[p]["samples_per_s"]) if accurate else "fp32"
+        s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision=lprec)
         s2.run(normals, uniforms)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -546,7 +563,7 @@ def main():
         t_plain_samp = n_samples / forms["fp64"]["samples_per_s"]
         learned_form = dict(rep)
         learned_form.update({
-            "precision": "fp32 contraction",
+            "precision": f"{lprec} contraction",
             "training_s": t_train,
             "construction_s": t_lc,
             "construction_speedup_vs_plain": t_build / t_lc,

@@ -612,6 +612,29 @@ __device__ __forceinline__ cd bfs_ld(const BfsData& bd, int slot, int64_t h) {
     return ld_cd(bd.data, (int64_t)slot * bd.stride + h);
 }
 
+// XCD-aware work mapping of the split path's (work item, block of 64 hafnians) grids.  The
+// hardware hands workgroups to the 8 XCDs round-robin by linear workgroup id, and each XCD has
+// its own L2; all work items of one hafnian block read that block's shared data (the pivot
+// polynomials of a RIGHT step, the gathered matrix of a subtree), so the W = n_items *
+// n_hblocks work units, ordered hafnian-block-major, are cut into 8 contiguous ranges of
+// Q = ceil(W / 8), one per XCD: linear id L runs on XCD c = L % 8 and takes unit c Q + L / 8.
+// A hafnian block is then split between at most two XCDs; the grid is 8 Q workgroups, of which
+// at most 7 have no unit.
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool xcd_work(int n_items, int n_hblocks, int& item, int& hblock) {
+    const int64_t W = (int64_t)n_items * n_hblocks;
+    const int64_t Q = (W + kXcds - 1) / kXcds;
+    const int64_t L = blockIdx.x;
+    const int64_t w = (L % kXcds) * Q + L / kXcds;
+    if (w >= W) return false;
+    hblock = (int)(w / n_items);
+    item = (int)(w % n_items);
+    return true;
+}
+inline int64_t xcd_grid(int n_items, int n_hblocks) {
+    return ((int64_t)n_items * n_hblocks + kXcds - 1) / kXcds * kXcds;
+}
+
 // Gather A and the root's g of every hafnian of the chunk (one lane per hafnian).
 template <int M, class S>
 __global__ __launch_bounds__(kWave) void haf_bfs_prep_kernel(S src, int64_t item0, BfsData bd, int rootg) {
@@ -699,14 +722,17 @@ __device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int
     }
 }
 
-// One tree level of the upper phase: blockIdx.x = item (op, entry) of this level,
-// blockIdx.y = block of 64 hafnians.
+// One tree level of the upper phase: workgroup = (item (op, entry) of this level, block of 64
+// hafnians), placed by xcd_work.
 template <int M>
 __global__ __launch_bounds__(kWave) void haf_bfs_level_kernel(BfsData bd, const BfsOp* __restrict__ ops,
-                                                              const int2* __restrict__ items) {
-    const int64_t h = (int64_t)blockIdx.y * kWave + threadIdx.x;
+                                                              const int2* __restrict__ items, int n_items,
+                                                              int n_hblocks) {
+    int item, hb;
+    if (!xcd_work(n_items, n_hblocks, item, hb)) return;
+    const int64_t h = (int64_t)hb * kWave + threadIdx.x;
     if (h >= bd.n_h) return;
-    const int2 it = items[blockIdx.x];
+    const int2 it = items[item];
     const BfsOp op = ops[it.x];
     // the plan only makes degrees 0, 1, 3, 7, 15 and M (checked on the host); M first, since
     // it can coincide with 7 or 15
@@ -725,15 +751,19 @@ struct LdsAOff {  // A(J, K) of a subtree whose rows start at OFF, [entry][lane]
 };
 
 // Frontier node: {k | D << 8 | G << 16 | (w > 0) << 24, po, plo | pcap << 8, g slot}.
+// Workgroup = (frontier node of this shape, block of 64 hafnians), placed by xcd_work.
 template <int M, int K0, int D>
 __global__ __launch_bounds__(kWave) void haf_subtree_kernel(BfsData bd, const int4* __restrict__ front,
-                                                            const int* __restrict__ tlist, double* __restrict__ vals) {
+                                                            const int* __restrict__ tlist, double* __restrict__ vals,
+                                                            int n_front, int n_hblocks) {
     constexpr int S0 = 2 * (M - K0);
     constexpr int NA = S0 * (S0 - 1) / 2;
     __shared__ cd lds[NA * kWave];
     const int lane = threadIdx.x;
-    const int64_t h = (int64_t)blockIdx.x * kWave + lane;
-    const int f = tlist[blockIdx.y];
+    int fi, hb;
+    if (!xcd_work(n_front, n_hblocks, fi, hb)) return;
+    const int64_t h = (int64_t)hb * kWave + lane;
+    const int f = tlist[fi];
     const int4 fr = front[f];
     if (h >= bd.n_h) return;
 #pragma unroll
@@ -997,10 +1027,12 @@ int get_fplan(int device, int m, FrontierPlan** out) {
 }
 
 template <int M>
-int launch_subtrees_m(hipStream_t st, int K0, int D, dim3 grid, const BfsData& bd, const int4* front,
-                      const int* tlist, double* vals) {
-#define BSMPS_SUBTREE(KK, DD) \
-    hipLaunchKernelGGL((haf_subtree_kernel<M, KK, DD>), grid, dim3(kWave), 0, st, bd, front, tlist, vals)
+int launch_subtrees_m(hipStream_t st, int K0, int D, int n_front, int n_hblocks, const BfsData& bd,
+                      const int4* front, const int* tlist, double* vals) {
+    const dim3 grid((unsigned)xcd_grid(n_front, n_hblocks));
+#define BSMPS_SUBTREE(KK, DD)                                                                              \
+    hipLaunchKernelGGL((haf_subtree_kernel<M, KK, DD>), grid, dim3(kWave), 0, st, bd, front, tlist, vals, \
+                       n_front, n_hblocks)
     if (K0 == M - 4 && D == 0) BSMPS_SUBTREE(M - 4, 0);
     else if (K0 == M - 4 && D == 1) BSMPS_SUBTREE(M - 4, 1);
     else if (K0 == M - 3 && D == 3) BSMPS_SUBTREE(M - 3, 3);
@@ -1024,6 +1056,18 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
     int64_t H = (int64_t)(ctx->split_chunk_bytes / per_h);
     H = std::max<int64_t>(kWave, H / kWave * kWave);
     H = std::min<int64_t>(H, (src.n_items + kWave - 1) / kWave * kWave);
+    // one launch covers (items of a level or subtree shape) x (hafnian blocks) workgroups of
+    // 64 lanes, which must stay below 2^32 threads
+    int max_items = 1;
+    for (int c : fp->level_count) max_items = std::max(max_items, c);
+    for (int c : fp->type_count) max_items = std::max(max_items, c);
+    const int64_t max_hblocks = (((int64_t)1 << 32) / kWave - kXcds) / max_items;
+    if (max_hblocks < 1) {
+        bsmps_set_error("frontier-split hafnian of size %d: %d work items per launch exceed the grid limit", 2 * m,
+                        max_items);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    H = std::min<int64_t>(H, max_hblocks * kWave);
     void* scratch = nullptr;
     BSMPS_TRY(bsmps_ctx_scratch(ctx, per_h * (size_t)H, &scratch));
     double* data = reinterpret_cast<double*>(scratch);
@@ -1039,14 +1083,16 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
                            fp->rootg);                                                                        \
         BSMPS_HIP(hipGetLastError());                                                                         \
         for (size_t l = 0; l < fp->level_first.size(); ++l) {                                                 \
-            hipLaunchKernelGGL((haf_bfs_level_kernel<MM>), dim3((unsigned)fp->level_count[l], hblocks), dim3(kWave), \
-                               0, ctx->stream, bd, fp->d_ops, fp->d_items + fp->level_first[l]);              \
+            hipLaunchKernelGGL((haf_bfs_level_kernel<MM>),                                                    \
+                               dim3((unsigned)xcd_grid(fp->level_count[l], (int)hblocks)), dim3(kWave), 0,    \
+                               ctx->stream, bd, fp->d_ops, fp->d_items + fp->level_first[l],                  \
+                               fp->level_count[l], (int)hblocks);                                             \
             BSMPS_HIP(hipGetLastError());                                                                     \
         }                                                                                                     \
         for (size_t t = 0; t < fp->types.size(); ++t)                                                         \
-            BSMPS_TRY(launch_subtrees_m<MM>(ctx->stream, fp->types[t].x, fp->types[t].y,                      \
-                                            dim3(hblocks, fp->type_count[t]), bd, fp->d_front,                \
-                                            fp->d_tlist + fp->type_first[t], vals));                          \
+            BSMPS_TRY(launch_subtrees_m<MM>(ctx->stream, fp->types[t].x, fp->types[t].y, fp->type_count[t],   \
+                                            (int)hblocks, bd, fp->d_front, fp->d_tlist + fp->type_first[t],   \
+                                            vals));                                                           \
         break;
             BSMPS_SPLIT_CASE(6)
             BSMPS_SPLIT_CASE(7)

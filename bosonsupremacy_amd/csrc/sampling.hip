@@ -517,6 +517,182 @@ __global__ __launch_bounds__(256) void site_gemm_bf16x3_kernel(int64_t n, int ch
 }
 
 // ---------------------------------------------------------------------------------------
+// BF16x6 form of the contraction: FP32-accurate products on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Every FP32 value x is split exactly into
+// three BF16 parts x = h + m + l (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m); both
+// differences are exact in FP32, so h + m + l carries all 24 significand bits), and a product
+// keeps the six terms of relative order >= 2^-16: hh + hm + mh + hl + mm + lh (the dropped
+// ones are ~2^-24, the size of an FP32 rounding).  The complex product runs on planes:
+// Re = Ar Br + (-Ai) Bi, Im = Ar Bi + Ai Br.  Both operands arrive split, as six bf16 planes
+// (parts h, m, l of the real part, then of the imaginary part) with k contiguous and padded
+// to a multiple of 32: lambda_prev * Gamma once per site (bsmps_split_bf16x6, [part][col][k]),
+// the carried vectors once per step (bsmps_split_rows_bf16x6, [part][row][k]), so the GEMM
+// itself does no conversion.  Workgroup tile 128 rows x 64 complex columns x 32 complex k,
+// staged in LDS with an XOR swizzle of the 16-byte k groups (conflict-free fragment reads);
+// four waves of 64 rows x 32 columns = 4 x 2 blocks of 16 x 16, each with a real and an
+// imaginary accumulator.
+// ---------------------------------------------------------------------------------------
+constexpr int X6_TM = 128, X6_TN = 64, X6_KC = 32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+    h = (__bf16)x;
+    const float r1 = x - (float)h;
+    m = (__bf16)r1;
+    l = (__bf16)(r1 - (float)m);
+}
+
+// 16-byte group g (of 4) of LDS row r is stored in slot g ^ f(r)
+__device__ __forceinline__ int x6_slot(int r, int g) { return g ^ (((r >> 2) ^ (r >> 1)) & 3); }
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, v) ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u,
+                                                                          0x80008000u});
+}
+
+// acc += a * b over the six retained part products (parts ordered h, m, l)
+__device__ __forceinline__ float4_t mfma6(const bf16x8 (&a)[3], const bf16x8* b, float4_t acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+    return acc;
+}
+
+// planes[p][c][k] of lambda[k] * gamma[k][c]; c < ncols_pad, k < kpad (zero padded)
+__global__ void split_bf16x6_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
+                                    int ncols, int kpad, int ncols_pad, __bf16* __restrict__ planes) {
+    const int64_t total = (int64_t)ncols_pad * kpad, plane = total;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / kpad), k = (int)(i % kpad);
+        float2 g = make_float2(0.f, 0.f);
+        if (c < ncols && k < chi_prev) {
+            g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
+            const double l = lam[k];
+            g.x = (float)(g.x * l);
+            g.y = (float)(g.y * l);
+        }
+        split3(g.x, planes[i], planes[plane + i], planes[2 * plane + i]);
+        split3(g.y, planes[3 * plane + i], planes[4 * plane + i], planes[5 * plane + i]);
+    }
+}
+
+// planes[p][r][k] of pre[r][k] (complex64 [n][chi_prev]); k < kpad (zero padded)
+__global__ void split_rows_bf16x6_kernel(const float* __restrict__ pre, int64_t n, int chi_prev, int kpad,
+                                         __bf16* __restrict__ planes) {
+    const int64_t total = n * kpad, plane = total;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / kpad;
+        const int k = (int)(i % kpad);
+        const float2 v = k < chi_prev ? reinterpret_cast<const float2*>(pre)[r * chi_prev + k] : make_float2(0.f, 0.f);
+        split3(v.x, planes[i], planes[plane + i], planes[2 * plane + i]);
+        split3(v.y, planes[3 * plane + i], planes[4 * plane + i], planes[5 * plane + i]);
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                               const __bf16* __restrict__ apl,
+                                                               const __bf16* __restrict__ bpl,
+                                                               float* __restrict__ T) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[6 * X6_TM * X6_KC];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[6 * X6_TN * X6_KC];
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int64_t row0 = (int64_t)blockIdx.y * X6_TM;
+    const int col0 = blockIdx.x * X6_TN;  // complex column
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 32;
+    const int64_t a_plane = n * kpad, b_plane = (int64_t)ncols_pad * kpad;
+
+    float4_t acc_re[4][2], acc_im[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    // staging pieces of 16 bytes: A 6 planes x 128 rows x 4 groups (12 per thread, fetched into
+    // registers one chunk ahead), B 6 planes x 64 columns x 4 groups (6 per thread, loaded while
+    // staging: the B tile is shared by every row block, so it is usually an L2 hit); rows past n
+    // read row n - 1 (their results are not stored)
+    u32x4 ra[12];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 12; ++u) {
+            const int q = tid + 256 * u, p = q >> 9, r = (q >> 2) & 127, g = q & 3;
+            const int64_t srow = row0 + r < n ? row0 + r : n - 1;
+            ra[u] = *reinterpret_cast<const u32x4*>(apl + p * a_plane + srow * kpad + k0 + 8 * g);
+        }
+    };
+    auto stash = [&](int k0) {
+        u32x4 rb[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int q = tid + 256 * u, p = q >> 8, c = (q >> 2) & 63, g = q & 3;
+            rb[u] = *reinterpret_cast<const u32x4*>(bpl + p * b_plane + (int64_t)(col0 + c) * kpad + k0 + 8 * g);
+        }
+#pragma unroll
+        for (int u = 0; u < 12; ++u) {
+            const int q = tid + 256 * u, p = q >> 9, r = (q >> 2) & 127, g = q & 3;
+            *reinterpret_cast<u32x4*>(As + ((p * X6_TM + r) * 4 + x6_slot(r, g)) * 8) = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            const int q = tid + 256 * u, p = q >> 8, c = (q >> 2) & 63, g = q & 3;
+            *reinterpret_cast<u32x4*>(Bs + ((p * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rb[u];
+        }
+    };
+
+    const int g = lane / 16;
+    fetch(0);
+    for (int k0 = 0; k0 < kpad; k0 += X6_KC) {
+        stash(k0);
+        __syncthreads();
+        if (k0 + X6_KC < kpad) fetch(k0 + X6_KC);
+        // fragments: lane holds k 8 (lane / 16) .. + 8 of row / column lane % 16
+        bf16x8 bfr[2][6];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = wn + 16 * j + lane % 16;
+#pragma unroll
+            for (int p = 0; p < 6; ++p)
+                bfr[j][p] = *reinterpret_cast<const bf16x8*>(Bs + ((p * X6_TN + c) * 4 + x6_slot(c, g)) * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = wm + 16 * i + lane % 16;
+            bf16x8 ar[3], ai[3], nai[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                ar[t] = *reinterpret_cast<const bf16x8*>(As + ((t * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+                ai[t] = *reinterpret_cast<const bf16x8*>(As + (((t + 3) * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+                nai[t] = neg8(ai[t]);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc_re[i][j] = mfma6(ar, &bfr[j][0], acc_re[i][j]);
+                acc_re[i][j] = mfma6(nai, &bfr[j][3], acc_re[i][j]);
+                acc_im[i][j] = mfma6(ar, &bfr[j][3], acc_im[i][j]);
+                acc_im[i][j] = mfma6(ai, &bfr[j][0], acc_im[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    // output register r of lane l holds row 4 (l / 16) + r, column l % 16 of each block
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t srow = row0 + wm + 16 * i + 4 * (lane / 16) + r;
+                const int c = col0 + wn + 16 * j + lane % 16;
+                if (srow < n && c < ncols)
+                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(acc_re[i][j][r], acc_im[i][j][r]);
+            }
+}
+
+// ---------------------------------------------------------------------------------------
 // Displacement + probabilities + draw, one workgroup per sample.
 // ---------------------------------------------------------------------------------------
 template <typename TT>
@@ -922,6 +1098,82 @@ extern "C" int bsmps_site_contract_bf16x3(bsmps_ctx* ctx, int64_t n, int chi_pre
     hipLaunchKernelGGL(site_gemm_bf16x3_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
                        chi_prev, (int)ncols, pre, lambda_prev, reinterpret_cast<const __bf16*>(gamma_hi),
                        reinterpret_cast<const __bf16*>(gamma_lo), T);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+// padded plane extents of the BF16x6 form
+static int64_t x6_kpad(int chi_prev) { return ((int64_t)chi_prev + X6_KC - 1) / X6_KC * X6_KC; }
+static int64_t x6_cols(int64_t ncols) { return (ncols + X6_TN - 1) / X6_TN * X6_TN; }
+
+extern "C" int64_t bsmps_bf16x6_plane_bytes(int chi_prev, int chi, int d) {
+    if (chi_prev <= 0 || chi <= 0 || d <= 0) return -1;
+    return 6 * x6_kpad(chi_prev) * x6_cols((int64_t)chi * d) * (int64_t)sizeof(__bf16);
+}
+
+extern "C" int64_t bsmps_bf16x6_row_plane_bytes(int64_t n, int chi_prev) {
+    if (n < 0 || chi_prev <= 0) return -1;
+    return 6 * n * x6_kpad(chi_prev) * (int64_t)sizeof(__bf16);
+}
+
+extern "C" int bsmps_split_bf16x6(bsmps_ctx* ctx, int chi_prev, int chi, int d, const float* gamma,
+                                  const double* lambda_prev, void* planes) {
+    if (!ctx || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_split_bf16x6: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (!gamma || !lambda_prev || !planes) {
+        bsmps_set_error("bsmps_split_bf16x6: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
+    if (ncols_pad > 0x7fffffff) {
+        bsmps_set_error("bsmps_split_bf16x6: Gamma too large");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(split_bf16x6_kernel, dim3(grid_for(kpad * ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
+                       lambda_prev, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad, reinterpret_cast<__bf16*>(planes));
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_split_rows_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, const float* pre, void* planes) {
+    if (!ctx || n < 0 || chi_prev <= 0) {
+        bsmps_set_error("bsmps_split_rows_bf16x6: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre || !planes) {
+        bsmps_set_error("bsmps_split_rows_bf16x6: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t kpad = x6_kpad(chi_prev);
+    hipLaunchKernelGGL(split_rows_bf16x6_kernel, dim3(grid_for(n * kpad, 256)), dim3(256), 0, ctx->stream, pre, n,
+                       chi_prev, (int)kpad, reinterpret_cast<__bf16*>(planes));
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                          const void* pre_planes, const void* gamma_planes, float* T) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_site_contract_bf16x6: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre_planes || !gamma_planes || !T) {
+        bsmps_set_error("bsmps_site_contract_bf16x6: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
+    const int64_t gx = ncols_pad / X6_TN, gy = (n + X6_TM - 1) / X6_TM;
+    if (gx > 0x7fffffff || gy > 65535 || ncols > 0x7fffffff) {
+        bsmps_set_error("bsmps_site_contract_bf16x6: problem too large for one launch");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(site_gemm_bf16x6_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+                       (int)ncols, (int)kpad, (int)ncols_pad, reinterpret_cast<const __bf16*>(pre_planes),
+                       reinterpret_cast<const __bf16*>(gamma_planes), T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }

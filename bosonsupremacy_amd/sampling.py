@@ -99,7 +99,7 @@ class Sampler:
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
     """
 
-    PRECISIONS = ("fp64", "fp32", "bf16x3")
+    PRECISIONS = ("fp64", "fp32", "bf16x3", "bf16x6")
 
     def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64", ctx=None):
         self.ctx = ctx or runtime.context()
@@ -120,6 +120,17 @@ class Sampler:
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         self.precision = precision
+        # device preparation runs on the context's stream (after the caller's pending work), so a
+        # context bound to a side stream sees its inputs ready
+        self.ctx.stream.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(self.ctx.stream):
+            self._prepare(gammas, lambdas, sqrtW, dd, dev)
+        self.dev = dev
+
+    def _prepare(self, gammas, lambdas, sqrtW, dd, dev):
+        """Device copies of the site tensors in the form's precision (and their bf16 splits)."""
+        precision = self.precision
+        self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
         ctype = torch.complex128 if precision == "fp64" else torch.complex64
         self.gammas = [g.to(ctype).contiguous() for g in gammas]
         self.splits = None
@@ -132,12 +143,21 @@ class Sampler:
                 lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
                 self.ctx.call("bsmps_split_bf16", runtime.ptr(g), cnt, runtime.ptr(hi), runtime.ptr(lo))
                 self.splits.append((hi, lo))
-        self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
+        if precision == "bf16x6":
+            # six bf16 planes of every contracted Lambda_{i-1} Gamma_i (FP32-exact split), reused
+            # by every batch
+            self.splits = [None]
+            for i in range(1, self.M):
+                chi_p, chi, d = self.chis[i - 1], self.chis[i], self.ds[i]
+                nbytes = self.ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d)
+                planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
+                self.ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(self.gammas[i]),
+                              runtime.ptr(self.lambdas[i - 1]), runtime.ptr(planes))
+                self.splits.append(planes)
         self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
         W, x = displacement_basis(dd)
         self.W = runtime.to_device(W)
         self.x = runtime.to_device(x)
-        self.dev = dev
 
     def run(self, normals, uniforms, return_probs=False, timer=None):
         """normals [2M, n], uniforms [M, n] (device float64) -> samples [n, M] int32 (device),
@@ -157,6 +177,10 @@ class Sampler:
         pre = torch.empty((n * self.chi,), dtype=ctype, device=self.dev)
         T = torch.empty((n * max(c * dk for c, dk in zip(self.chis, self.ds)),), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
+        pre_planes = None
+        if self.precision == "bf16x6":
+            nbytes = max(self.ctx.lib.bsmps_bf16x6_row_plane_bytes(n, c) for c in self.chis)
+            pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=self.dev)
         for i in range(M):
             last = i == M - 1
             chi, d = self.chis[i], self.ds[i]
@@ -171,6 +195,10 @@ class Sampler:
                     elif self.precision == "fp32":
                         ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre),
                                  runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                    elif self.precision == "bf16x6":
+                        ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre), runtime.ptr(pre_planes))
+                        ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(pre_planes),
+                                 runtime.ptr(self.splits[i]), runtime.ptr(T))
                     else:
                         hi, lo = self.splits[i]
                         ctx.call("bsmps_site_contract_bf16x3", n, chi_p, chi, d, runtime.ptr(pre),

@@ -209,6 +209,26 @@ int bsmps_split_bf16(bsmps_ctx *ctx, const float *x, int64_t count, void *hi, vo
 int bsmps_site_contract_bf16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const float *pre,
                                const double *lambda_prev, const void *gamma_hi, const void *gamma_lo, float *T);
 
+/* BF16x6 form of the contraction, FP32-accurate on bf16 MFMA (no reference counterpart):
+ * each FP32 value is split exactly into three bf16 parts h + m + l and a product keeps the
+ * six terms hh + hm + mh + hl + mm + lh (dropped terms ~2^-24 relative), FP32 accumulation.
+ * Both operands are passed split, as six bf16 planes (parts h, m, l of the real part, then of
+ * the imaginary part) with k contiguous and padded with zeros to kpad = round_up(chi_prev, 32):
+ *   bsmps_split_bf16x6: lambda_prev[k] * gamma[k][c] (gamma complex64 [chi_prev][chi*d]) as
+ *     planes [6][round_up(chi*d, 64)][kpad], once per site; bsmps_bf16x6_plane_bytes gives
+ *     their size;
+ *   bsmps_split_rows_bf16x6: the carried vectors pre (complex64 [n][chi_prev]) as planes
+ *     [6][n][kpad], once per step; bsmps_bf16x6_row_plane_bytes gives their size.
+ * The size functions return -1 for invalid extents.  T is complex64 [n][chi*d] as in
+ * bsmps_site_contract_c64. */
+int64_t bsmps_bf16x6_plane_bytes(int chi_prev, int chi, int d);
+int64_t bsmps_bf16x6_row_plane_bytes(int64_t n, int chi_prev);
+int bsmps_split_bf16x6(bsmps_ctx *ctx, int chi_prev, int chi, int d, const float *gamma,
+                       const double *lambda_prev, void *planes);
+int bsmps_split_rows_bf16x6(bsmps_ctx *ctx, int64_t n, int chi_prev, const float *pre, void *planes);
+int bsmps_site_contract_bf16x6(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const void *pre_planes,
+                               const void *gamma_planes, float *T);
+
 /* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
  * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:
  * counter (e_lo, e_hi, stream, 0) with e = first+i, key (seed_lo, seed_hi).  A uniform is

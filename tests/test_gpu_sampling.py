@@ -189,3 +189,69 @@ def test_sampling_bf16x3_form_m8():
     same = np.all(a == b, axis=1)
     assert same.mean() > 0.98
     assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-3
+
+
+def _bf16x6_contract(ctx, pre, lam, G):
+    from bosonsupremacy_amd import runtime
+    n, chi_p = pre.shape
+    _, chi, d = G.shape
+    pre_d, lam_d, G_d = runtime.to_device(pre), runtime.to_device(lam), runtime.to_device(G)
+    kpad, cpad = -(-chi_p // 32) * 32, -(-(chi * d) // 64) * 64
+    nbytes = ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d)
+    assert nbytes == 6 * 2 * kpad * cpad
+    rbytes = ctx.lib.bsmps_bf16x6_row_plane_bytes(n, chi_p)
+    assert rbytes == 6 * 2 * n * kpad
+    planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=ctx.device)
+    rows = torch.empty((rbytes // 2,), dtype=torch.bfloat16, device=ctx.device)
+    ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G_d), runtime.ptr(lam_d), runtime.ptr(planes))
+    ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre_d), runtime.ptr(rows))
+    # the three parts of every value add up to it exactly
+    parts = rows.view(6, n, kpad).float()
+    re = (parts[0].double() + parts[1].double() + parts[2].double())[:, :chi_p]
+    im = (parts[3].double() + parts[4].double() + parts[5].double())[:, :chi_p]
+    assert torch.equal(re.float(), pre_d.real) and torch.equal(im.float(), pre_d.imag)
+    T = torch.empty((n, chi, d), dtype=torch.complex64, device=ctx.device)
+    ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(planes), runtime.ptr(T))
+    return runtime.to_host(T).astype(np.complex128)
+
+
+def test_site_contract_bf16x6_matches_matmul():
+    """FP32-accurate BF16x6 form: every entry within the FP32 GEMM error bound
+    K * 2^-24 * sum |a| |b| of the exact product, on ragged shapes (rows, k and columns not
+    multiples of the tile) and on operands spanning twelve decades."""
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(9)
+    for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
+                                     (129, 100, 30, 6, 6)):
+        scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
+        scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
+        pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
+        lam = rng.uniform(0.1, 1, size=chi_p)
+        G = ((rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))) * scale_g).astype(np.complex64)
+        A = pre.astype(np.complex128) * lam
+        B = G.astype(np.complex128).reshape(chi_p, chi * d)
+        want = (A @ B).reshape(n, chi, d)
+        bound = ((np.abs(A.real) + np.abs(A.imag)) @ (np.abs(B.real) + np.abs(B.imag))).reshape(n, chi, d)
+        got = _bf16x6_contract(ctx, pre, lam, G)
+        ratio = (np.abs(got - want) / bound).max()
+        print(f"bf16x6 {n}x{chi_p}x{chi * d} spread {spread}: max err / bound = {ratio:.2e}", flush=True)
+        assert ratio <= 2 * chi_p * 2.0 ** -24, (n, chi_p, chi, d)
+
+
+def test_sampling_bf16x6_form_m8():
+    """BF16x6 form: conditional distributions within 1e-5 of the FP64 form on the same random
+    numbers (the FP32 form's bar), the same draws away from cdf boundaries."""
+    from bosonsupremacy_amd import runtime, sampling
+    inst, G, L = _mps_m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 3000
+    rng = np.random.default_rng(6)
+    normals = runtime.to_device(rng.normal(size=(2 * inst.M, n)))
+    uniforms = runtime.to_device(rng.uniform(size=(inst.M, n)))
+    a, pa = sampling.Sampler(G, L, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
+    b, pb = sampling.Sampler(G, L, sqrtW, inst.dd, precision="bf16x6").run(normals, uniforms, return_probs=True)
+    a, b, pa, pb = (runtime.to_host(x) for x in (a, b, pa, pb))
+    same = np.all(a == b, axis=1)
+    assert same.mean() > 0.99
+    assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-5
