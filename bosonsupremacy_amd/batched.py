@@ -8,13 +8,13 @@ numbers for a batch are generated into the graph's static input buffers before e
 Launch shapes come from the project's autotuner (``autotune.load_or_tune``).
 
 ``run_batch`` times, per instance, the construction and one graph-replayed sampling pass in
-each of the FP64 and FP32 contraction forms (instances/s for either form), then measures the
-FP32 form's error against FP64 on the same sample paths: the FP32 pass is replayed with each
-uniform moved to the middle of the FP64 cumulative-probability interval of the FP64 draw, so
-both passes follow the FP64 samples (unless the two cdfs differ by half a probability, which
-is counted), and the Rao-Blackwellised single-mode marginals P(n_k = j) -- the mean over
-samples of the conditional distributions -- are compared.  It also reports how many draws
-differ when both forms use the same uniforms.
+the FP64 form and in each reduced-precision form (FP32, BF16x6 by default; instances/s for
+each), then measures every reduced form's error against FP64 on the same sample paths: its
+pass is replayed with each uniform moved to the middle of the FP64 cumulative-probability
+interval of the FP64 draw, so both passes follow the FP64 samples (unless the two cdfs differ
+by half a probability, which is counted), and the Rao-Blackwellised single-mode marginals
+P(n_k = j) -- the mean over samples of the conditional distributions -- are compared.  It also
+reports how many draws differ when both forms use the same uniforms.
 """
 import sys
 import time
@@ -69,17 +69,21 @@ def _mid_uniforms(samples, probs):
     return (cdf.gather(2, idx) - 0.5 * probs.gather(2, idx)).squeeze(2).contiguous()
 
 
-def run_batch(instances, n_samples=10 ** 4, verbose=False):
+def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6"), verbose=False):
     """Construct and sample every instance; returns a summary dict (progress lines go to
-    stderr when ``verbose``)."""
+    stderr when ``verbose``).  Each form in ``forms`` is timed and checked against FP64."""
     from ._lib import BsmpsError
 
     def now():
         torch.cuda.synchronize()
         return time.perf_counter()
     t_start = time.perf_counter()
-    tot = {"construction": 0.0, "setup_fp64": 0.0, "setup_fp32": 0.0, "sampling_fp64": 0.0, "sampling_fp32": 0.0}
-    abs_errs, rows, skipped = [], [], []
+    names = ("fp64",) + tuple(forms)
+    tot = {"construction": 0.0}
+    for f in names:
+        tot["setup_" + f] = tot["sampling_" + f] = 0.0
+    errs = {f: [] for f in forms}
+    rows, skipped = [], []
     for i, inst in enumerate(instances):
         t0 = now()
         try:
@@ -91,43 +95,52 @@ def run_batch(instances, n_samples=10 ** 4, verbose=False):
                 print(f"[batched] instance {i} skipped: {e}", file=sys.stderr, flush=True)
             continue
         t1 = now()
+        times = {"construction": t1 - t0}
         sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
-        g64 = GraphSampler(G, L, sqrtW, inst.dd, n_samples, "fp64", return_probs=True)
-        t2 = now()
-        g32 = GraphSampler(G, L, sqrtW, inst.dd, n_samples, "fp32", return_probs=True)
-        t3 = now()
+        graphs = {}
+        for f in names:
+            ta = now()
+            graphs[f] = GraphSampler(G, L, sqrtW, inst.dd, n_samples, f, return_probs=True)
+            times["setup_" + f] = now() - ta
         normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=inst.seed)
-        t4 = now()
-        s64, p64 = g64.run(normals, uniforms)
-        t5 = now()
-        s32, _ = g32.run(normals, uniforms)
-        t6 = now()
-        flips = int((s32 != s64).any(dim=1).sum())          # samples whose draws differ (same uniforms)
-        s32, p32 = g32.run(normals, _mid_uniforms(s64, p64))  # FP32 along the FP64 sample paths
-        off_path = int((s32 != s64).any(dim=1).sum())
-        abs_err = float(torch.abs(p32.mean(dim=1) - p64.mean(dim=1)).max())
-        times = {"construction": t1 - t0, "setup_fp64": t2 - t1, "setup_fp32": t3 - t2,
-                 "sampling_fp64": t5 - t4, "sampling_fp32": t6 - t5}
+        row = {"M": inst.M, "d": inst.d, "chi": inst.chi}
+        ta = now()
+        s64, p64 = graphs["fp64"].run(normals, uniforms)
+        times["sampling_fp64"] = now() - ta
+        mid = _mid_uniforms(s64, p64)
+        m64 = p64.mean(dim=1)
+        line = []
+        for f in forms:
+            ta = now()
+            sf, _ = graphs[f].run(normals, uniforms)
+            times["sampling_" + f] = now() - ta
+            flips = int((sf != s64).any(dim=1).sum())          # samples whose draws differ (same uniforms)
+            sf, pf = graphs[f].run(normals, mid)                # along the FP64 sample paths
+            off_path = int((sf != s64).any(dim=1).sum())
+            err = float(torch.abs(pf.mean(dim=1) - m64).max())
+            errs[f].append(err)
+            row.update({f"{f}_marginal_abs_err": err, f"{f}_draws_differing_same_uniforms": flips,
+                        f"{f}_off_fp64_path": off_path})
+            line.append(f"{f} {times['sampling_' + f]:.3f} s err {err:.1e} flips {flips}")
         for k, v in times.items():
             tot[k] += v
-        abs_errs.append(abs_err)
-        rows.append({"M": inst.M, "d": inst.d, "chi": inst.chi, **{k + "_s": v for k, v in times.items()},
-                     "fp32_marginal_abs_err": abs_err, "fp32_draws_differing_same_uniforms": flips,
-                     "fp32_off_fp64_path": off_path})
+        row.update({k + "_s": v for k, v in times.items()})
+        rows.append(row)
         if verbose:
             print(f"[batched] instance {i}: M={inst.M} d={inst.d} chi={inst.chi} construction {t1 - t0:.2f} s "
-                  f"sampling fp64 {t5 - t4:.3f} s fp32 {t6 - t5:.3f} s marginal err {abs_err:.2e} "
-                  f"differing draws {flips}/{n_samples}", file=sys.stderr, flush=True)
-        del G, L, g64, g32
+                  f"fp64 {times['sampling_fp64']:.3f} s " + " ".join(line), file=sys.stderr, flush=True)
+        del G, L, graphs
     wall = time.perf_counter() - t_start
     done = len(rows)
-    t64 = tot["construction"] + tot["setup_fp64"] + tot["sampling_fp64"]
-    t32 = tot["construction"] + tot["setup_fp32"] + tot["sampling_fp32"]
-    return {"instances": done, "skipped": skipped, "samples_per_instance": n_samples, "wall_s": wall,
-            "instances_per_s_fp64": done / t64, "instances_per_s_fp32": done / t32,
-            **{k + "_s": v for k, v in tot.items()},
-            "fp32_marginal_abs_err_max": float(np.max(abs_errs)),
-            "fp32_marginal_abs_err_median": float(np.median(abs_errs)),
-            "fp32_off_fp64_path_total": int(sum(r["fp32_off_fp64_path"] for r in rows)),
-            "fp32_draws_differing_same_uniforms_total": int(sum(r["fp32_draws_differing_same_uniforms"] for r in rows)),
-            "per_instance": rows}
+    out = {"instances": done, "skipped": skipped, "samples_per_instance": n_samples, "wall_s": wall}
+    for f in names:
+        out[f"instances_per_s_{f}"] = done / (tot["construction"] + tot["setup_" + f] + tot["sampling_" + f])
+    out.update({k + "_s": v for k, v in tot.items()})
+    for f in forms:
+        out[f"{f}_marginal_abs_err_max"] = float(np.max(errs[f]))
+        out[f"{f}_marginal_abs_err_median"] = float(np.median(errs[f]))
+        out[f"{f}_off_fp64_path_total"] = int(sum(r[f"{f}_off_fp64_path"] for r in rows))
+        out[f"{f}_draws_differing_same_uniforms_total"] = int(
+            sum(r[f"{f}_draws_differing_same_uniforms"] for r in rows))
+    out["per_instance"] = rows
+    return out
