@@ -97,8 +97,8 @@ def site_hafnian_work_by_site(kres, knum, d, err_tol=1e-10):
 
 def probe_sampling(inst, ko, probe, n, tuning):
     """Time one site's sampling step at full size: the contraction of n carried vectors with
-    a real Gamma_k (FP64 and FP32 forms) and the displacement + draw, then extrapolate the
-    chain's samples/s as n / ((M - 1) x per-site time)."""
+    a real Gamma_k (FP64, FP32 and BF16x6 forms) and the displacement + draw, then extrapolate
+    the chain's samples/s as n / ((M - 1) x per-site time)."""
     import torch
     from bosonsupremacy_amd import runtime, sampling
     ctx = runtime.context()
@@ -138,6 +138,19 @@ def probe_sampling(inst, ko, probe, n, tuning):
     T32 = torch.empty((n, chi, d), dtype=torch.complex64, device=dev)
     t32 = timed(lambda: ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre32),
                                  runtime.ptr(lam_prev), runtime.ptr(G32), runtime.ptr(T32)))
+    # BF16x6: Gamma split once per site (not timed, as in the sampler); the carried vectors are
+    # split once per step (timed with the contraction)
+    planes = torch.empty((ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.bfloat16, device=dev)
+    ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G32), runtime.ptr(lam_prev), runtime.ptr(planes))
+    del G32
+    rows = torch.empty((ctx.lib.bsmps_bf16x6_row_plane_bytes(n, chi_p) // 2,), dtype=torch.bfloat16, device=dev)
+
+    def x6():
+        ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre32), runtime.ptr(rows))
+        ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(planes),
+                 runtime.ptr(T32))
+    t6 = timed(x6)
+    del planes, rows
     W, x = sampling.displacement_basis(inst.dd)
     Wd, xd = runtime.to_device(W), runtime.to_device(x)
     alpha = (torch.randn((n, inst.M), dtype=torch.complex128, device=dev, generator=rng) * 0.3).contiguous()
@@ -150,11 +163,15 @@ def probe_sampling(inst, ko, probe, n, tuning):
     forms = {k: tuning.get(k, "blocks") for k in ("contract_fp64", "contract_fp32")}
     f64 = (6.0 if forms["contract_fp64"] == "gauss" else 8.0) * n * chi_p * chi * d  # MFMA flops executed
     f32 = (6.0 if forms["contract_fp32"] == "gauss" else 8.0) * n * chi_p * chi * d
+    useful = 8.0 * n * chi_p * chi * d  # complex-GEMM flops
     out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_fp64_s": t64, "contract_fp32_s": t32,
-           "sample_site_s": ts, "contract_forms": forms, "contract_fp64_tflops": f64 / t64 / 1e12,
-           "contract_fp32_tflops": f32 / t32 / 1e12,
+           "contract_bf16x6_s": t6, "sample_site_s": ts, "contract_forms": forms,
+           "contract_fp64_tflops": f64 / t64 / 1e12, "contract_fp32_tflops": f32 / t32 / 1e12,
+           "contract_bf16x6_complex_gemm_tflops": useful / t6 / 1e12,
+           "contract_bf16x6_mfma_utilisation": 6 * useful / t6 / 1e12 / PEAK_BF16_MFMA_TFLOPS,
            "chain_samples_per_s_fp64": n / ((inst.M - 1) * (t64 + ts)),
-           "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts))}
+           "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts)),
+           "chain_samples_per_s_bf16x6": n / ((inst.M - 1) * (t6 + ts))}
     log(f"sampling probe at site {k}: {out}")
     return out
 

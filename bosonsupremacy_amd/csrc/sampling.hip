@@ -592,16 +592,44 @@ __global__ void split_rows_bf16x6_kernel(const float* __restrict__ pre, int64_t 
     }
 }
 
+// Tile order of the GEMM grid.  Workgroups reach the 8 XCDs round-robin by linear id and each
+// XCD has its own L2, so XCD x takes the column blocks [x gx / 8, (x + 1) gx / 8) and walks its
+// (row block, column block) tiles in groups of 8 x 8: the ~64 workgroups an XCD runs at once then
+// share 8 row blocks of A and 8 column blocks of B in its L2, whatever the aspect ratio of the
+// product (n much smaller than chi d at chi = 10^4, or the reverse).  The grid is 8 x the
+// largest per-XCD tile count; workgroups past an XCD's count return at once.
+constexpr int kGemmXcds = 8, kTileGroup = 8;
+__device__ __forceinline__ bool gemm_tile(int gx, int gy, int& rb, int& cb) {
+    const int x = (int)(blockIdx.x % kGemmXcds);
+    const int64_t s = blockIdx.x / kGemmXcds;
+    const int c_lo = (int)((int64_t)x * gx / kGemmXcds), c_hi = (int)((int64_t)(x + 1) * gx / kGemmXcds);
+    const int nc = c_hi - c_lo;
+    if (nc <= 0 || s >= (int64_t)nc * gy) return false;
+    const int64_t rg = s / ((int64_t)kTileGroup * nc);        // group of 8 row blocks
+    const int R = min(kTileGroup, gy - (int)rg * kTileGroup);
+    const int64_t off = s - rg * kTileGroup * nc;              // < R * nc
+    const int cg = (int)(off / (R * kTileGroup));              // group of 8 column blocks
+    const int off2 = (int)(off - (int64_t)cg * R * kTileGroup);  // < R * min(8, nc - 8 cg)
+    rb = (int)rg * kTileGroup + off2 % R;
+    cb = c_lo + cg * kTileGroup + off2 / R;
+    return true;
+}
+inline int64_t gemm_tile_grid(int64_t gx, int64_t gy) {
+    return kGemmXcds * ((gx + kGemmXcds - 1) / kGemmXcds) * gy;
+}
+
 __global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
                                                                const __bf16* __restrict__ apl,
                                                                const __bf16* __restrict__ bpl,
                                                                float* __restrict__ T) {
     __shared__ __attribute__((aligned(16))) __bf16 As[6 * X6_TM * X6_KC];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[6 * X6_TN * X6_KC];
+    int rb, cb;
+    if (!gemm_tile(ncols_pad / X6_TN, (int)((n + X6_TM - 1) / X6_TM), rb, cb)) return;
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
-    const int64_t row0 = (int64_t)blockIdx.y * X6_TM;
-    const int col0 = blockIdx.x * X6_TN;  // complex column
+    const int64_t row0 = (int64_t)rb * X6_TM;
+    const int col0 = cb * X6_TN;  // complex column
     const int wm = (wave / 2) * 64, wn = (wave % 2) * 32;
     const int64_t a_plane = n * kpad, b_plane = (int64_t)ncols_pad * kpad;
 
@@ -1167,11 +1195,11 @@ extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_pre
     }
     const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
     const int64_t gx = ncols_pad / X6_TN, gy = (n + X6_TM - 1) / X6_TM;
-    if (gx > 0x7fffffff || gy > 65535 || ncols > 0x7fffffff) {
+    if (gy > 0x7fffffff || ncols > 0x7fffffff || gemm_tile_grid(gx, gy) > ((int64_t)1 << 32) / 256 - 1) {
         bsmps_set_error("bsmps_site_contract_bf16x6: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_bf16x6_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+    hipLaunchKernelGGL(site_gemm_bf16x6_kernel, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0, ctx->stream, n,
                        (int)ncols, (int)kpad, (int)ncols_pad, reinterpret_cast<const __bf16*>(pre_planes),
                        reinterpret_cast<const __bf16*>(gamma_planes), T);
     BSMPS_HIP(hipGetLastError());
