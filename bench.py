@@ -176,6 +176,38 @@ def probe_sampling(inst, ko, probe, n, tuning):
     return out
 
 
+def hafnian_only(args):
+    """Batched hafnians of one size on synthetic targets drawn from a random 64 x 64 Sigma,
+    timed with HIP events over ``--steps`` calls (for profiling the hafnian kernels alone; the
+    library's default launch shapes, no autotuning run, so a profile holds only these calls)."""
+    import torch
+    from bosonsupremacy_amd import runtime
+    n = args.hafnian_size
+    ctx = runtime.context()
+    rng = np.random.default_rng(0)
+    ld = 64
+    S = runtime.to_device((rng.normal(size=(ld, ld)) + 1j * rng.normal(size=(ld, ld))) * 0.3)
+    count = args.hafnian_count or max(1000, int(3e10 / hafnian_flops(n // 2)))
+    tg = runtime.to_device(rng.integers(0, ld, size=(count, n)).astype(np.int32))
+    out = torch.empty((count,), dtype=torch.complex128, device=ctx.device)
+
+    def run():
+        ctx.call("bsmps_hafnian_targets", runtime.ptr(S), ld, runtime.ptr(tg), count, n, None, runtime.ptr(out))
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        run()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    print(json.dumps({"hafnian_size": n, "count": count, "ms_per_call": ms,
+                      "tflops": hafnian_flops(n // 2) * count / (ms * 1e-3) / 1e12,
+                      "launch_shape": "split" if n >= 12 else "lane" if n <= 8 else "group:64"}))
+
+
 def partial_run(args):
     """Part of an instance too large for one session (the M=144, chi=10^4 workload): the
     Schmidt spectra of every cut, the exact hafnian work of every site, and the listed sites
@@ -368,6 +400,10 @@ def main():
                     help="with --partial-sites including M//2: time that site's sampling step for this many samples")
     ap.add_argument("--batched", type=int, default=0,
                     help="run the batched form over this many random instances (graph-captured sampling)")
+    ap.add_argument("--hafnian-size", type=int, default=0,
+                    help="only time batched hafnians of this size on synthetic targets (the launch shape "
+                         "the autotuner picks; for profiling the hafnian kernels alone)")
+    ap.add_argument("--hafnian-count", type=int, default=0, help="hafnians per call with --hafnian-size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -379,6 +415,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": x.numel() * 8, "copies": 3}))
+        return
+    if args.hafnian_size:
+        hafnian_only(args)
         return
     if args.no_forms:
         args.no_learned = True
