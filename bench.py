@@ -157,8 +157,10 @@ def probe_sampling(inst, ko, probe, n, tuning):
     u = torch.rand((n,), dtype=torch.float64, device=dev, generator=rng)
     smp = torch.empty((n, 1), dtype=torch.int32, device=dev)
     nxt = torch.empty((n, chi), dtype=torch.complex128, device=dev)
+    dwork = torch.empty((2 * n * inst.dd * d,), dtype=torch.float64, device=dev)
     ts = timed(lambda: ctx.call("bsmps_sample_site", n, chi, d, inst.dd, runtime.ptr(T), chi * d,
-                                runtime.ptr(alpha), inst.M, runtime.ptr(Wd), runtime.ptr(xd), runtime.ptr(lam),
+                                runtime.ptr(alpha), inst.M, runtime.ptr(Wd), runtime.ptr(xd), runtime.ptr(dwork),
+                                runtime.ptr(lam),
                                 runtime.ptr(u), runtime.ptr(smp), 1, runtime.ptr(nxt), None))
     forms = {k: tuning.get(k, "blocks") for k in ("contract_fp64", "contract_fp32")}
     f64 = (6.0 if forms["contract_fp64"] == "gauss" else 8.0) * n * chi_p * chi * d  # MFMA flops executed

@@ -68,12 +68,13 @@ _PROTOTYPES = {
     "bsmps_site_contract": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
                             c_int),
     "bsmps_sample_site": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
-                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p], c_int),
+                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
+                          c_int),
     "bsmps_site_contract_c64": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p], c_int),
     "bsmps_sample_site_c64": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
-                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
-                              c_int),
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                               c_void_p], c_int),
     "bsmps_split_bf16": ([c_void_p, c_void_p, c_int64, c_void_p, c_void_p], c_int),
     "bsmps_site_contract_bf16x3": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p], c_int),

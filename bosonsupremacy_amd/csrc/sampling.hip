@@ -742,33 +742,30 @@ __device__ __forceinline__ void st_c(float* p, int64_t i, cd v) {
 constexpr int kMaxDD = 64;
 constexpr int kMaxd = 32;
 
-// DMAX / DDMAX bound d and dd at compile time so the per-thread arrays stay in VGPRs.
-// One wave per sample, four samples per workgroup.  TT: storage type of T and pre (double:
-// FP64 form; float: reduced-precision forms, where the carried vector is rescaled by its
-// largest component after every site so that a long chain cannot underflow -- the
-// reference's GPU path does the same, distributed_sampling.py:147).  The displacement, the
-// probabilities and the inverse CDF are evaluated in FP64 in every form.
-template <int DMAX, int DDMAX, typename TT, bool RENORM>
-__global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
-                                                          const TT* __restrict__ T, int64_t tstride,
-                                                          const double* __restrict__ alpha, int64_t astride,
-                                                          const double* __restrict__ W, const double* __restrict__ x,
-                                                          const double* __restrict__ lam,
-                                                          const double* __restrict__ uni, int32_t* __restrict__ samples,
-                                                          int64_t sstride, TT* __restrict__ pre,
-                                                          double* __restrict__ probs) {
+// Orders LDS writes of one wave before its later LDS reads (the waves of a workgroup work on
+// separate samples and separate LDS slices, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// D(alpha_s) of every sample of one site, Dout[s][k][j] (k < dd, j < d, complex128): one wave per
+// sample, cos/sin of |alpha| x_q in LDS (the spectral form of bsmps_batch_displaces).  Kept out
+// of the draw kernel, whose occupancy the FP64 sincos registers would halve.
+template <int DDMAX>
+__global__ __launch_bounds__(256) void site_displace_kernel(int64_t n, int d, int dd, const double* __restrict__ alpha,
+                                                            int64_t astride, const double* __restrict__ W,
+                                                            const double* __restrict__ x, double* __restrict__ Dout) {
     __shared__ double s_cr[4][DDMAX], s_sr[4][DDMAX];
-    __shared__ cd s_D[4][DDMAX * DMAX];  // D[k][j], j < d
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     double* cr = s_cr[wave];
     double* sr = s_sr[wave];
-    cd* Dm = s_D[wave];
     const int64_t n_rounds = (n + 3) / 4;
     for (int64_t rnd = blockIdx.x; rnd < n_rounds; rnd += gridDim.x) {
         const int64_t s = rnd * 4 + wave;
-        const bool live = s < n;   // uniform per wave
-        const int64_t se = live ? s : n - 1;
-        const cd a = ld_cd(alpha, se * astride);
+        if (s >= n) continue;  // uniform per wave
+        const cd a = ld_cd(alpha, s * astride);
         const double r = hypot(a.re, a.im);
         const double theta = atan2(a.im, a.re) + 1.5707963267948966;
         if (lane < dd) {
@@ -777,7 +774,7 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
             cr[lane] = cs;
             sr[lane] = -sn;  // e^{-i r x_q}
         }
-        __syncthreads();
+        wave_lds_sync();
         for (int e = lane; e < dd * d; e += 64) {
             const int k = e / d, j = e - k * d;
             double re = 0.0, im = 0.0;
@@ -788,14 +785,50 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
             }
             double sp, cp;
             sincos((double)(k - j) * theta, &sp, &cp);
-            Dm[k * DMAX + j] = cd{re * cp - im * sp, re * sp + im * cp};
+            st_cd(Dout, s * dd * d + e, cd{re * cp - im * sp, re * sp + im * cp});
         }
-        __syncthreads();
+        wave_lds_sync();
+    }
+}
+
+// DMAX / DDMAX bound d and dd at compile time so the per-thread arrays stay in VGPRs (the
+// tightest of a few bounds is launched: fewer registers, more waves per SIMD).
+// One wave per sample, four samples per workgroup, each wave independent; D(alpha_s) comes from
+// site_displace_kernel.  TT: storage type of T and pre (double: FP64 form; float:
+// reduced-precision forms, where the carried vector is rescaled by its largest component after
+// every site so that a long chain cannot underflow -- the reference's GPU path does the same,
+// distributed_sampling.py:147).  The probabilities and the inverse CDF are evaluated in FP64 in
+// every form.
+template <int DMAX, int DDMAX, typename TT, bool RENORM>
+__global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
+                                                          const TT* __restrict__ T, int64_t tstride,
+                                                          const double* __restrict__ Dg,
+                                                          const double* __restrict__ lam,
+                                                          const double* __restrict__ uni, int32_t* __restrict__ samples,
+                                                          int64_t sstride, TT* __restrict__ pre,
+                                                          double* __restrict__ probs) {
+    __shared__ cd s_D[4][DDMAX * DMAX];  // D[k][j], j < d
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    cd* Dm = s_D[wave];
+    const int64_t n_rounds = (n + 3) / 4;
+    for (int64_t rnd = blockIdx.x; rnd < n_rounds; rnd += gridDim.x) {
+        const int64_t s = rnd * 4 + wave;
+        const bool live = s < n;   // uniform per wave
+        const int64_t se = live ? s : n - 1;
+        for (int e = lane; e < dd * d; e += 64) {
+            const int k = e / d, j = e - k * d;
+            Dm[k * DMAX + j] = ld_cd(Dg, se * dd * d + e);
+        }
+        wave_lds_sync();
         const TT* Ts = T + 2 * se * tstride;
-        // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda)
-        double pk[DDMAX];
+        // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda);
+        // with RENORM also mx[k] = max_a max(|Re|, |Im|) of the candidate row sum_j D[k][j] T[a][j],
+        // so the rescaled carried vector needs one more pass over T, not two
+        double pk[DDMAX], mx[RENORM ? DDMAX : 1];
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) pk[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < (RENORM ? DDMAX : 1); ++k) mx[k] = 0.0;
         const int a_end = lam ? chi : 1;
         for (int aa = lane; aa < a_end; aa += 64) {
             cd t[DMAX];
@@ -815,16 +848,22 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                         }
                     }
                     pk[k] = fma(fma(ur, ur, ui * ui), l2, pk[k]);
+                    if constexpr (RENORM) mx[k] = fmax(mx[k], fmax(fabs(ur), fabs(ui)));
                 }
             }
         }
-        // wave reduction; lane 0 ends with the totals
+        // wave reduction; lane 0 ends with the totals (every lane with the maxima)
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) {
             if (k < dd) {
                 double v = pk[k];
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
                 pk[k] = v;
+                if constexpr (RENORM) {
+                    double m = mx[k];
+                    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+                    mx[k] = m;
+                }
             }
         }
         int pick = 0;
@@ -862,13 +901,11 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
             };
             double scale = 1.0;
             if constexpr (RENORM) {
-                double mx = 0.0;
-                for (int aa = lane; aa < chi; aa += 64) {
-                    const cd u = sel(aa);
-                    mx = fmax(mx, fmax(fabs(u.re), fabs(u.im)));
-                }
-                for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-                scale = mx > 0.0 ? 1.0 / mx : 1.0;
+                double m = 0.0;
+#pragma unroll
+                for (int k = 0; k < DDMAX; ++k)
+                    if (k == pick) m = mx[k];
+                scale = m > 0.0 ? 1.0 / m : 1.0;
             }
             for (int aa = lane; aa < chi; aa += 64) {
                 cd u = sel(aa);
@@ -876,7 +913,7 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                 st_c(pre, s * chi + aa, u);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
@@ -993,7 +1030,7 @@ extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int 
 
 extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const double* T,
                                  int64_t T_sample_stride, const double* alpha, int64_t alpha_stride,
-                                 const double* eigvec, const double* eigval, const double* lambda,
+                                 const double* eigvec, const double* eigval, double* dwork, const double* lambda,
                                  const double* uniforms, int32_t* samples, int64_t samples_stride, double* pre,
                                  double* probs) {
     if (!ctx || n < 0 || chi <= 0 || d <= 0 || d > kMaxd || dd < d || dd > kMaxDD || T_sample_stride < 0 ||
@@ -1002,15 +1039,24 @@ extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int 
         return BSMPS_ERR_INVALID;
     }
     if (n == 0) return BSMPS_OK;
-    if (!T || !alpha || !eigvec || !eigval || !uniforms || !samples) {
+    if (!T || !alpha || !eigvec || !eigval || !dwork || !uniforms || !samples) {
         bsmps_set_error("bsmps_sample_site: null pointer");
         return BSMPS_ERR_INVALID;
     }
-#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                        \
-    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, double, false>), dim3(grid_for((n + 3) / 4, 1)), dim3(256), 0, \
-                       ctx->stream, n, chi, d, dd, T, T_sample_stride, alpha, alpha_stride, eigvec, eigval,      \
-                       lambda, uniforms, samples, samples_stride, pre, probs)
-    if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+    const dim3 grid(grid_for((n + 3) / 4, 1)), block(256);
+    if (dd <= 16)
+        hipLaunchKernelGGL(site_displace_kernel<16>, grid, block, 0, ctx->stream, n, d, dd, alpha, alpha_stride, eigvec,
+                           eigval, dwork);
+    else
+        hipLaunchKernelGGL(site_displace_kernel<kMaxDD>, grid, block, 0, ctx->stream, n, d, dd, alpha, alpha_stride,
+                           eigvec, eigval, dwork);
+    BSMPS_HIP(hipGetLastError());
+#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
+    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, double, false>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
+                       T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
+    if (d <= 4 && dd <= 8) BSMPS_SAMPLE_LAUNCH(4, 8);
+    else if (d <= 6 && dd <= 12) BSMPS_SAMPLE_LAUNCH(6, 12);
+    else if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
 #undef BSMPS_SAMPLE_LAUNCH
@@ -1064,7 +1110,7 @@ extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, 
 
 extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const float* T,
                                      int64_t T_sample_stride, const double* alpha, int64_t alpha_stride,
-                                     const double* eigvec, const double* eigval, const double* lambda,
+                                     const double* eigvec, const double* eigval, double* dwork, const double* lambda,
                                      const double* uniforms, int32_t* samples, int64_t samples_stride, float* pre,
                                      double* probs) {
     if (!ctx || n < 0 || chi <= 0 || d <= 0 || d > kMaxd || dd < d || dd > kMaxDD || T_sample_stride < 0 ||
@@ -1073,15 +1119,24 @@ extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, 
         return BSMPS_ERR_INVALID;
     }
     if (n == 0) return BSMPS_OK;
-    if (!T || !alpha || !eigvec || !eigval || !uniforms || !samples) {
+    if (!T || !alpha || !eigvec || !eigval || !dwork || !uniforms || !samples) {
         bsmps_set_error("bsmps_sample_site_c64: null pointer");
         return BSMPS_ERR_INVALID;
     }
-#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                        \
-    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, float, true>), dim3(grid_for((n + 3) / 4, 1)), dim3(256), 0,   \
-                       ctx->stream, n, chi, d, dd, T, T_sample_stride, alpha, alpha_stride, eigvec, eigval,      \
-                       lambda, uniforms, samples, samples_stride, pre, probs)
-    if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+    const dim3 grid(grid_for((n + 3) / 4, 1)), block(256);
+    if (dd <= 16)
+        hipLaunchKernelGGL(site_displace_kernel<16>, grid, block, 0, ctx->stream, n, d, dd, alpha, alpha_stride, eigvec,
+                           eigval, dwork);
+    else
+        hipLaunchKernelGGL(site_displace_kernel<kMaxDD>, grid, block, 0, ctx->stream, n, d, dd, alpha, alpha_stride,
+                           eigvec, eigval, dwork);
+    BSMPS_HIP(hipGetLastError());
+#define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
+    hipLaunchKernelGGL((sample_site_kernel<DM, DDM, float, true>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
+                       T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
+    if (d <= 4 && dd <= 8) BSMPS_SAMPLE_LAUNCH(4, 8);
+    else if (d <= 6 && dd <= 12) BSMPS_SAMPLE_LAUNCH(6, 12);
+    else if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
 #undef BSMPS_SAMPLE_LAUNCH

@@ -177,6 +177,7 @@ class Sampler:
         pre = torch.empty((n * self.chi,), dtype=ctype, device=self.dev)
         T = torch.empty((n * max(c * dk for c, dk in zip(self.chis, self.ds)),), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
+        dwork = torch.empty((2 * n * dd * self.d,), dtype=torch.float64, device=self.dev)  # D_s per sample
         pre_planes = None
         if self.precision == "bf16x6":
             nbytes = max(self.ctx.lib.bsmps_bf16x6_row_plane_bytes(n, c) for c in self.chis)
@@ -209,7 +210,8 @@ class Sampler:
             p_ptr = runtime.ptr(probs[i]) if probs is not None else None
             with timer("sample_site"):
                 ctx.call(draw, n, chi, d, dd, runtime.ptr(src), stride, a_ptr, M,
-                         runtime.ptr(self.W), runtime.ptr(self.x), None if last else runtime.ptr(self.lambdas[i]),
+                         runtime.ptr(self.W), runtime.ptr(self.x), runtime.ptr(dwork),
+                         None if last else runtime.ptr(self.lambdas[i]),
                          runtime.ptr(uniforms[i]), s_ptr, M, None if last else runtime.ptr(pre), p_ptr)
         return (samples, probs) if return_probs else samples
 

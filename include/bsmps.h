@@ -179,10 +179,12 @@ int bsmps_site_contract(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d,
  *   n_s      = #{k : u_s > cumsum(p / sum p)[k]}, clamped to dd-1
  *   pre[s][a] = U[a][n_s]
  * T is read at T + s*T_sample_stride (complex elements; stride 0 broadcasts one [chi][d]
- * tensor, as the first site does).  probs (may be NULL) receives p / sum p [n][dd]. */
+ * tensor, as the first site does).  probs (may be NULL) receives p / sum p [n][dd].
+ * dwork: caller-owned scratch for the n displacements, complex128 [n][dd][d] (16*n*dd*d
+ * bytes); D_s is computed into it by a first launch and read by the draw. */
 int bsmps_sample_site(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, const double *T,
                       int64_t T_sample_stride, const double *alpha, int64_t alpha_stride,
-                      const double *eigvec, const double *eigval, const double *lambda,
+                      const double *eigvec, const double *eigval, double *dwork, const double *lambda,
                       const double *uniforms, int32_t *samples, int64_t samples_stride,
                       double *pre, double *probs);
 
@@ -196,7 +198,7 @@ int bsmps_site_contract_c64(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, in
                             const double *lambda_prev, const float *gamma, float *T);
 int bsmps_sample_site_c64(bsmps_ctx *ctx, int64_t n, int chi, int d, int dd, const float *T,
                           int64_t T_sample_stride, const double *alpha, int64_t alpha_stride,
-                          const double *eigvec, const double *eigval, const double *lambda,
+                          const double *eigvec, const double *eigval, double *dwork, const double *lambda,
                           const double *uniforms, int32_t *samples, int64_t samples_stride, float *pre,
                           double *probs);
 
