@@ -83,6 +83,12 @@ _PROTOTYPES = {
     "bsmps_split_bf16x6": ([c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "bsmps_split_rows_bf16x6": ([c_void_p, c_int64, c_int, c_void_p, c_void_p], c_int),
     "bsmps_site_contract_bf16x6": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "bsmps_fp16x3_plane_bytes": ([c_int, c_int, c_int], c_int64),
+    "bsmps_fp16x3_row_plane_bytes": ([c_int64, c_int], c_int64),
+    "bsmps_split_fp16x3": ([c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "bsmps_split_rows_fp16x3": ([c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "bsmps_site_contract_fp16x3": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p], c_int),
     "bsmps_philox": ([c_void_p, c_uint64, c_uint32, c_int, c_int64, c_int64, c_int, c_void_p], c_int),
     "bsmps_mlp_forward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 10, c_int),
     "bsmps_mlp_backward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 13, c_int),

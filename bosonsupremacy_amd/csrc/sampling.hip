@@ -545,20 +545,48 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // 16-byte group g (of 4) of LDS row r is stored in slot g ^ f(r)
 __device__ __forceinline__ int x6_slot(int r, int g) { return g ^ (((r >> 2) ^ (r >> 1)) & 3); }
 
-__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
-    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, v) ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u,
-                                                                          0x80008000u});
+// Operand formats of the split-precision contraction.  An FP32 value is split into NP parts
+// of a 16-bit type; prod() adds the retained part products of one real product to acc.
+struct Bf16x6Fmt {  // three bf16 parts (exact), products of order >= 2^-16: six MFMAs
+    typedef __bf16 E;
+    typedef bf16x8 V;
+    static constexpr int NP = 3;
+    static constexpr bool SCALED = false;
+    static constexpr int MIN_WAVES = 2;  // waves per SIMD the registers are budgeted for
+    __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+    __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
+        acc = mfma(a[2], b[0], acc);
+        acc = mfma(a[1], b[1], acc);
+        acc = mfma(a[0], b[2], acc);
+        acc = mfma(a[1], b[0], acc);
+        acc = mfma(a[0], b[1], acc);
+        return mfma(a[0], b[0], acc);
+    }
+};
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+struct Fp16x3Fmt {  // two fp16 parts (22 bits) of values scaled to max 1: three MFMAs
+    typedef _Float16 E;
+    typedef half8 V;
+    static constexpr int NP = 2;
+    static constexpr bool SCALED = true;
+    static constexpr int MIN_WAVES = 2;  // at 3 the kernel spills
+    __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+    __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
+        acc = mfma(a[1], b[0], acc);
+        acc = mfma(a[0], b[1], acc);
+        return mfma(a[0], b[0], acc);
+    }
+};
+
+template <class V>
+__device__ __forceinline__ V neg16x8(V v) {  // sign flip of 8 16-bit floats (bf16 or fp16)
+    return __builtin_bit_cast(V, __builtin_bit_cast(u32x4, v) ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u,
+                                                                      0x80008000u});
 }
 
-// acc += a * b over the six retained part products (parts ordered h, m, l)
-__device__ __forceinline__ float4_t mfma6(const bf16x8 (&a)[3], const bf16x8* b, float4_t acc) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
-    return acc;
+__device__ __forceinline__ void split2_fp16(float x, _Float16& h, _Float16& l) {
+    h = (_Float16)x;
+    l = (_Float16)(x - (float)h);
 }
 
 // planes[p][c][k] of lambda[k] * gamma[k][c]; c < ncols_pad, k < kpad (zero padded)
@@ -618,12 +646,21 @@ inline int64_t gemm_tile_grid(int64_t gx, int64_t gy) {
     return kGemmXcds * ((gx + kGemmXcds - 1) / kGemmXcds) * gy;
 }
 
-__global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
-                                                               const __bf16* __restrict__ apl,
-                                                               const __bf16* __restrict__ bpl,
-                                                               float* __restrict__ T) {
-    __shared__ __attribute__((aligned(16))) __bf16 As[6 * X6_TM * X6_KC];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[6 * X6_TN * X6_KC];
+// One kernel for both split formats: F::NP parts per value, 2 F::NP planes per operand (parts of
+// the real part, then of the imaginary part).  With F::SCALED the planes hold values divided by a
+// per-row (A) and per-column (B) scale, which the epilogue multiplies back.
+template <class F>
+__global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                                 const typename F::E* __restrict__ apl,
+                                                                 const float* __restrict__ rowscale,
+                                                                 const typename F::E* __restrict__ bpl,
+                                                                 const float* __restrict__ colscale,
+                                                                 float* __restrict__ T) {
+    typedef typename F::E E;
+    typedef typename F::V V;
+    constexpr int NP = F::NP, NPL = 2 * F::NP;
+    __shared__ __attribute__((aligned(16))) E As[NPL * X6_TM * X6_KC];
+    __shared__ __attribute__((aligned(16))) E Bs[NPL * X6_TN * X6_KC];
     int rb, cb;
     if (!gemm_tile(ncols_pad / X6_TN, (int)((n + X6_TM - 1) / X6_TM), rb, cb)) return;
     const int tid = threadIdx.x;
@@ -639,35 +676,35 @@ __global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-    // staging pieces of 16 bytes: A 6 planes x 128 rows x 4 groups (12 per thread, fetched into
-    // registers one chunk ahead), B 6 planes x 64 columns x 4 groups (6 per thread, loaded while
+    // staging pieces of 16 bytes: A NPL planes x 128 rows x 4 groups (2 NPL per thread, fetched into
+    // registers one chunk ahead), B NPL planes x 64 columns x 4 groups (NPL per thread, loaded while
     // staging: the B tile is shared by every row block, so it is usually an L2 hit); rows past n
     // read row n - 1 (their results are not stored)
-    u32x4 ra[12];
+    u32x4 ra[2 * NPL];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 12; ++u) {
-            const int q = tid + 256 * u, p = q >> 9, r = (q >> 2) & 127, g = q & 3;
+        for (int u = 0; u < 2 * NPL; ++u) {
+            const int q = tid + 256 * u, pl = q >> 9, r = (q >> 2) & 127, g = q & 3;
             const int64_t srow = row0 + r < n ? row0 + r : n - 1;
-            ra[u] = *reinterpret_cast<const u32x4*>(apl + p * a_plane + srow * kpad + k0 + 8 * g);
+            ra[u] = *reinterpret_cast<const u32x4*>(apl + pl * a_plane + srow * kpad + k0 + 8 * g);
         }
     };
     auto stash = [&](int k0) {
-        u32x4 rb[6];
+        u32x4 rb[NPL];
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            const int q = tid + 256 * u, p = q >> 8, c = (q >> 2) & 63, g = q & 3;
-            rb[u] = *reinterpret_cast<const u32x4*>(bpl + p * b_plane + (int64_t)(col0 + c) * kpad + k0 + 8 * g);
+        for (int u = 0; u < NPL; ++u) {
+            const int q = tid + 256 * u, pl = q >> 8, c = (q >> 2) & 63, g = q & 3;
+            rb[u] = *reinterpret_cast<const u32x4*>(bpl + pl * b_plane + (int64_t)(col0 + c) * kpad + k0 + 8 * g);
         }
 #pragma unroll
-        for (int u = 0; u < 12; ++u) {
-            const int q = tid + 256 * u, p = q >> 9, r = (q >> 2) & 127, g = q & 3;
-            *reinterpret_cast<u32x4*>(As + ((p * X6_TM + r) * 4 + x6_slot(r, g)) * 8) = ra[u];
+        for (int u = 0; u < 2 * NPL; ++u) {
+            const int q = tid + 256 * u, pl = q >> 9, r = (q >> 2) & 127, g = q & 3;
+            *reinterpret_cast<u32x4*>(As + ((pl * X6_TM + r) * 4 + x6_slot(r, g)) * 8) = ra[u];
         }
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            const int q = tid + 256 * u, p = q >> 8, c = (q >> 2) & 63, g = q & 3;
-            *reinterpret_cast<u32x4*>(Bs + ((p * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rb[u];
+        for (int u = 0; u < NPL; ++u) {
+            const int q = tid + 256 * u, pl = q >> 8, c = (q >> 2) & 63, g = q & 3;
+            *reinterpret_cast<u32x4*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rb[u];
         }
     };
 
@@ -678,30 +715,30 @@ __global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int
         __syncthreads();
         if (k0 + X6_KC < kpad) fetch(k0 + X6_KC);
         // fragments: lane holds k 8 (lane / 16) .. + 8 of row / column lane % 16
-        bf16x8 bfr[2][6];
+        V bfr[2][NPL];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int c = wn + 16 * j + lane % 16;
 #pragma unroll
-            for (int p = 0; p < 6; ++p)
-                bfr[j][p] = *reinterpret_cast<const bf16x8*>(Bs + ((p * X6_TN + c) * 4 + x6_slot(c, g)) * 8);
+            for (int pl = 0; pl < NPL; ++pl)
+                bfr[j][pl] = *reinterpret_cast<const V*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = wm + 16 * i + lane % 16;
-            bf16x8 ar[3], ai[3], nai[3];
+            V ar[NP], ai[NP], nai[NP];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                ar[t] = *reinterpret_cast<const bf16x8*>(As + ((t * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
-                ai[t] = *reinterpret_cast<const bf16x8*>(As + (((t + 3) * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
-                nai[t] = neg8(ai[t]);
+            for (int t = 0; t < NP; ++t) {
+                ar[t] = *reinterpret_cast<const V*>(As + ((t * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+                ai[t] = *reinterpret_cast<const V*>(As + (((t + NP) * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+                nai[t] = neg16x8(ai[t]);
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                acc_re[i][j] = mfma6(ar, &bfr[j][0], acc_re[i][j]);
-                acc_re[i][j] = mfma6(nai, &bfr[j][3], acc_re[i][j]);
-                acc_im[i][j] = mfma6(ar, &bfr[j][3], acc_im[i][j]);
-                acc_im[i][j] = mfma6(ai, &bfr[j][0], acc_im[i][j]);
+                acc_re[i][j] = F::prod(ar, &bfr[j][0], acc_re[i][j]);
+                acc_re[i][j] = F::prod(nai, &bfr[j][NP], acc_re[i][j]);
+                acc_im[i][j] = F::prod(ar, &bfr[j][NP], acc_im[i][j]);
+                acc_im[i][j] = F::prod(ai, &bfr[j][0], acc_im[i][j]);
             }
         }
         __syncthreads();
@@ -715,9 +752,74 @@ __global__ __launch_bounds__(256, 2) void site_gemm_bf16x6_kernel(int64_t n, int
             for (int r = 0; r < 4; ++r) {
                 const int64_t srow = row0 + wm + 16 * i + 4 * (lane / 16) + r;
                 const int c = col0 + wn + 16 * j + lane % 16;
-                if (srow < n && c < ncols)
-                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(acc_re[i][j][r], acc_im[i][j][r]);
+                if (srow < n && c < ncols) {
+                    float re = acc_re[i][j][r], im = acc_im[i][j][r];
+                    if constexpr (F::SCALED) {
+                        const float sc = rowscale[srow] * colscale[c];
+                        re *= sc;
+                        im *= sc;
+                    }
+                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(re, im);
+                }
             }
+}
+
+// FP16x3 operand preparation.  Gamma: colscale[c] = max_k max(|Re|, |Im|) of lambda[k] gamma[k][c]
+// (1 for an all-zero column), then planes[p][c][k] of the two fp16 parts of that entry divided
+// by colscale[c].  Carried vectors: one wave per row, rowscale[r] = the row's largest component
+// (1 for a zero row), planes[p][r][k] of the parts of pre[r][k] / rowscale[r].
+__global__ void colscale_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
+                                       int ncols, int ncols_pad, float* __restrict__ colscale) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncols_pad; c += gridDim.x * blockDim.x) {
+        float m = 0.f;
+        if (c < ncols)
+            for (int k = 0; k < chi_prev; ++k) {
+                const float2 g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
+                const double l = lam[k];
+                m = fmaxf(m, fmaxf(fabsf((float)(g.x * l)), fabsf((float)(g.y * l))));
+            }
+        colscale[c] = m > 0.f ? m : 1.f;
+    }
+}
+
+__global__ void split_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam,
+                                    const float* __restrict__ colscale, int chi_prev, int ncols, int kpad, int ncols_pad,
+                                    _Float16* __restrict__ planes) {
+    const int64_t total = (int64_t)ncols_pad * kpad, plane = total;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / kpad), k = (int)(i % kpad);
+        float2 g = make_float2(0.f, 0.f);
+        if (c < ncols && k < chi_prev) {
+            g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
+            const double l = lam[k];
+            const float sc = colscale[c];
+            g.x = (float)(g.x * l) / sc;
+            g.y = (float)(g.y * l) / sc;
+        }
+        split2_fp16(g.x, planes[i], planes[plane + i]);
+        split2_fp16(g.y, planes[2 * plane + i], planes[3 * plane + i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void split_rows_fp16x3_kernel(const float* __restrict__ pre, int64_t n, int chi_prev,
+                                                                int kpad, _Float16* __restrict__ planes,
+                                                                float* __restrict__ rowscale) {
+    const int lane = threadIdx.x % 64;
+    const int64_t plane = n * kpad;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; r < n; r += (int64_t)gridDim.x * 4) {
+        const float2* row = reinterpret_cast<const float2*>(pre) + r * chi_prev;
+        float m = 0.f;
+        for (int k = lane; k < chi_prev; k += 64) m = fmaxf(m, fmaxf(fabsf(row[k].x), fabsf(row[k].y)));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        const float sc = m > 0.f ? m : 1.f;
+        if (lane == 0) rowscale[r] = sc;
+        for (int k = lane; k < kpad; k += 64) {
+            const float2 v = k < chi_prev ? row[k] : make_float2(0.f, 0.f);
+            const int64_t i = r * kpad + k;
+            split2_fp16(v.x / sc, planes[i], planes[plane + i]);
+            split2_fp16(v.y / sc, planes[2 * plane + i], planes[3 * plane + i]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1254,9 +1356,88 @@ extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_pre
         bsmps_set_error("bsmps_site_contract_bf16x6: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_bf16x6_kernel, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0, ctx->stream, n,
-                       (int)ncols, (int)kpad, (int)ncols_pad, reinterpret_cast<const __bf16*>(pre_planes),
-                       reinterpret_cast<const __bf16*>(gamma_planes), T);
+    hipLaunchKernelGGL(site_gemm_split_kernel<Bf16x6Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0,
+                       ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
+                       reinterpret_cast<const __bf16*>(pre_planes), nullptr,
+                       reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int64_t bsmps_fp16x3_plane_bytes(int chi_prev, int chi, int d) {
+    if (chi_prev <= 0 || chi <= 0 || d <= 0) return -1;
+    return 4 * x6_kpad(chi_prev) * x6_cols((int64_t)chi * d) * (int64_t)sizeof(_Float16);
+}
+
+extern "C" int64_t bsmps_fp16x3_row_plane_bytes(int64_t n, int chi_prev) {
+    if (n < 0 || chi_prev <= 0) return -1;
+    return 4 * n * x6_kpad(chi_prev) * (int64_t)sizeof(_Float16);
+}
+
+extern "C" int bsmps_split_fp16x3(bsmps_ctx* ctx, int chi_prev, int chi, int d, const float* gamma,
+                                  const double* lambda_prev, void* planes, float* colscale) {
+    if (!ctx || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_split_fp16x3: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (!gamma || !lambda_prev || !planes || !colscale) {
+        bsmps_set_error("bsmps_split_fp16x3: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
+    if (ncols_pad > 0x7fffffff) {
+        bsmps_set_error("bsmps_split_fp16x3: Gamma too large");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(colscale_fp16x3_kernel, dim3(grid_for(ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
+                       lambda_prev, chi_prev, (int)ncols, (int)ncols_pad, colscale);
+    BSMPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(split_fp16x3_kernel, dim3(grid_for(kpad * ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
+                       lambda_prev, colscale, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad,
+                       reinterpret_cast<_Float16*>(planes));
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_split_rows_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, const float* pre, void* planes,
+                                       float* rowscale) {
+    if (!ctx || n < 0 || chi_prev <= 0) {
+        bsmps_set_error("bsmps_split_rows_fp16x3: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre || !planes || !rowscale) {
+        bsmps_set_error("bsmps_split_rows_fp16x3: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(split_rows_fp16x3_kernel, dim3(grid_for((n + 3) / 4, 1)), dim3(256), 0, ctx->stream, pre, n,
+                       chi_prev, (int)x6_kpad(chi_prev), reinterpret_cast<_Float16*>(planes), rowscale);
+    BSMPS_HIP(hipGetLastError());
+    return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                          const void* pre_planes, const float* rowscale, const void* gamma_planes,
+                                          const float* colscale, float* T) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
+        bsmps_set_error("bsmps_site_contract_fp16x3: invalid arguments");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return BSMPS_OK;
+    if (!pre_planes || !rowscale || !gamma_planes || !colscale || !T) {
+        bsmps_set_error("bsmps_site_contract_fp16x3: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
+    const int64_t gx = ncols_pad / X6_TN, gy = (n + X6_TM - 1) / X6_TM;
+    if (gy > 0x7fffffff || ncols > 0x7fffffff || gemm_tile_grid(gx, gy) > ((int64_t)1 << 32) / 256 - 1) {
+        bsmps_set_error("bsmps_site_contract_fp16x3: problem too large for one launch");
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(site_gemm_split_kernel<Fp16x3Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0,
+                       ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
+                       reinterpret_cast<const _Float16*>(pre_planes), rowscale,
+                       reinterpret_cast<const _Float16*>(gamma_planes), colscale, T);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }

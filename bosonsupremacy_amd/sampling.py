@@ -99,7 +99,7 @@ class Sampler:
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
     """
 
-    PRECISIONS = ("fp64", "fp32", "bf16x3", "bf16x6")
+    PRECISIONS = ("fp64", "fp32", "bf16x3", "bf16x6", "fp16x3")
 
     def __init__(self, gammas, lambdas, sqrtW, dd, precision="fp64", ctx=None):
         self.ctx = ctx or runtime.context()
@@ -154,6 +154,18 @@ class Sampler:
                 self.ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(self.gammas[i]),
                               runtime.ptr(self.lambdas[i - 1]), runtime.ptr(planes))
                 self.splits.append(planes)
+        if precision == "fp16x3":
+            # two fp16 planes per part of every contracted Lambda_{i-1} Gamma_i, columns scaled to a
+            # largest component of 1 (the scales are multiplied back by the contraction)
+            self.splits = [None]
+            for i in range(1, self.M):
+                chi_p, chi, d = self.chis[i - 1], self.chis[i], self.ds[i]
+                nbytes = self.ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d)
+                planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
+                colscale = torch.empty((-(-(chi * d) // 64) * 64,), dtype=torch.float32, device=dev)
+                self.ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(self.gammas[i]),
+                              runtime.ptr(self.lambdas[i - 1]), runtime.ptr(planes), runtime.ptr(colscale))
+                self.splits.append((planes, colscale))
         self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
         W, x = displacement_basis(dd)
         self.W = runtime.to_device(W)
@@ -178,10 +190,14 @@ class Sampler:
         T = torch.empty((n * max(c * dk for c, dk in zip(self.chis, self.ds)),), dtype=ctype, device=self.dev)
         probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
         dwork = torch.empty((2 * n * dd * self.d,), dtype=torch.float64, device=self.dev)  # D_s per sample
-        pre_planes = None
+        pre_planes = rowscale = None
         if self.precision == "bf16x6":
             nbytes = max(self.ctx.lib.bsmps_bf16x6_row_plane_bytes(n, c) for c in self.chis)
             pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=self.dev)
+        if self.precision == "fp16x3":
+            nbytes = max(self.ctx.lib.bsmps_fp16x3_row_plane_bytes(n, c) for c in self.chis)
+            pre_planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=self.dev)
+            rowscale = torch.empty((n,), dtype=torch.float32, device=self.dev)
         for i in range(M):
             last = i == M - 1
             chi, d = self.chis[i], self.ds[i]
@@ -196,6 +212,12 @@ class Sampler:
                     elif self.precision == "fp32":
                         ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre),
                                  runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
+                    elif self.precision == "fp16x3":
+                        planes, colscale = self.splits[i]
+                        ctx.call("bsmps_split_rows_fp16x3", n, chi_p, runtime.ptr(pre), runtime.ptr(pre_planes),
+                                 runtime.ptr(rowscale))
+                        ctx.call("bsmps_site_contract_fp16x3", n, chi_p, chi, d, runtime.ptr(pre_planes),
+                                 runtime.ptr(rowscale), runtime.ptr(planes), runtime.ptr(colscale), runtime.ptr(T))
                     elif self.precision == "bf16x6":
                         ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre), runtime.ptr(pre_planes))
                         ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(pre_planes),

@@ -231,6 +231,23 @@ int bsmps_split_rows_bf16x6(bsmps_ctx *ctx, int64_t n, int chi_prev, const float
 int bsmps_site_contract_bf16x6(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const void *pre_planes,
                                const void *gamma_planes, float *T);
 
+/* FP16x3 form of the contraction (no reference counterpart): values are scaled to a largest
+ * component of 1 -- per column of lambda_prev * gamma, per row of pre -- and split into two fp16
+ * parts h = fp16(x), l = fp16(x - h) (22 significand bits); a product keeps hh + hl + lh on fp16
+ * MFMA (v_mfma_f32_16x16x32_f16) with FP32 accumulation, and the epilogue multiplies the scales
+ * back (relative error ~2^-21 per product, three MFMAs per real product instead of six).
+ * Planes as in the BF16x6 form with two parts: [4][round_up(chi*d, 64)][kpad] (once per site,
+ * with colscale float [round_up(chi*d, 64)]) and [4][n][kpad] (once per step, with rowscale
+ * float [n]). */
+int64_t bsmps_fp16x3_plane_bytes(int chi_prev, int chi, int d);
+int64_t bsmps_fp16x3_row_plane_bytes(int64_t n, int chi_prev);
+int bsmps_split_fp16x3(bsmps_ctx *ctx, int chi_prev, int chi, int d, const float *gamma,
+                       const double *lambda_prev, void *planes, float *colscale);
+int bsmps_split_rows_fp16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, const float *pre, void *planes,
+                            float *rowscale);
+int bsmps_site_contract_fp16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const void *pre_planes,
+                               const float *rowscale, const void *gamma_planes, const float *colscale, float *T);
+
 /* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
  * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:
  * counter (e_lo, e_hi, stream, 0) with e = first+i, key (seed_lo, seed_hi).  A uniform is

@@ -255,3 +255,77 @@ def test_sampling_bf16x6_form_m8():
     same = np.all(a == b, axis=1)
     assert same.mean() > 0.99
     assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-5
+
+
+def _fp16x3_contract(ctx, pre, lam, G):
+    from bosonsupremacy_amd import runtime
+    n, chi_p = pre.shape
+    _, chi, d = G.shape
+    pre_d, lam_d, G_d = runtime.to_device(pre), runtime.to_device(lam), runtime.to_device(G)
+    kpad, cpad = -(-chi_p // 32) * 32, -(-(chi * d) // 64) * 64
+    assert ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d) == 4 * 2 * kpad * cpad
+    assert ctx.lib.bsmps_fp16x3_row_plane_bytes(n, chi_p) == 4 * 2 * n * kpad
+    planes = torch.empty((4 * kpad * cpad,), dtype=torch.float16, device=ctx.device)
+    colscale = torch.empty((cpad,), dtype=torch.float32, device=ctx.device)
+    rows = torch.empty((4 * n * kpad,), dtype=torch.float16, device=ctx.device)
+    rowscale = torch.empty((n,), dtype=torch.float32, device=ctx.device)
+    ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(G_d), runtime.ptr(lam_d), runtime.ptr(planes),
+             runtime.ptr(colscale))
+    ctx.call("bsmps_split_rows_fp16x3", n, chi_p, runtime.ptr(pre_d), runtime.ptr(rows), runtime.ptr(rowscale))
+    # the scales are the largest components, and the two parts of every scaled value add up to it
+    # within fp16's 22-bit two-part precision
+    want_rs = np.maximum(np.abs(pre.real), np.abs(pre.imag)).max(axis=1)
+    assert np.allclose(runtime.to_host(rowscale), want_rs, rtol=0, atol=0)
+    parts = rows.view(4, n, kpad).double()
+    re = (parts[0] + parts[1])[:, :chi_p] * rowscale.double()[:, None]
+    assert torch.abs(re - pre_d.real.double()).max() <= 2.0 ** -21 * float(want_rs.max())
+    T = torch.empty((n, chi, d), dtype=torch.complex64, device=ctx.device)
+    ctx.call("bsmps_site_contract_fp16x3", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(rowscale),
+             runtime.ptr(planes), runtime.ptr(colscale), runtime.ptr(T))
+    return runtime.to_host(T).astype(np.complex128)
+
+
+def test_site_contract_fp16x3_matches_matmul():
+    """FP16x3 form: every entry within K * 2^-19 * (row scale) * (column scale) of the exact
+    product (per product the error is ~2^-21 of the scaled operands), on ragged shapes and on
+    operands spanning twelve decades."""
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(10)
+    for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
+                                     (129, 100, 30, 6, 6)):
+        scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
+        scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
+        pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
+        lam = rng.uniform(0.1, 1, size=chi_p)
+        G = ((rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))) * scale_g).astype(np.complex64)
+        A = pre.astype(np.complex128)
+        B = G.astype(np.complex128).reshape(chi_p, chi * d) * lam[:, None]
+        want = (A @ B).reshape(n, chi, d)
+        rs = np.maximum(np.abs(A.real), np.abs(A.imag)).max(axis=1)
+        cs = np.maximum(np.abs(B.real), np.abs(B.imag)).max(axis=0)
+        bound = (chi_p * np.outer(rs, cs)).reshape(n, chi, d)
+        got = _fp16x3_contract(ctx, pre, lam, G)
+        ratio = (np.abs(got - want) / bound).max()
+        print(f"fp16x3 {n}x{chi_p}x{chi * d} spread {spread}: max err / (K rowscale colscale) = {ratio:.2e}",
+              flush=True)
+        assert ratio <= 2.0 ** -19, (n, chi_p, chi, d)
+
+
+def test_sampling_fp16x3_form_m8():
+    """FP16x3 form: conditional distributions within 1e-5 of the FP64 form on the same random
+    numbers, the same draws away from cdf boundaries."""
+    from bosonsupremacy_amd import runtime, sampling
+    inst, G, L = _mps_m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 3000
+    rng = np.random.default_rng(11)
+    normals = runtime.to_device(rng.normal(size=(2 * inst.M, n)))
+    uniforms = runtime.to_device(rng.uniform(size=(inst.M, n)))
+    a, pa = sampling.Sampler(G, L, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
+    b, pb = sampling.Sampler(G, L, sqrtW, inst.dd, precision="fp16x3").run(normals, uniforms, return_probs=True)
+    a, b, pa, pb = (runtime.to_host(x) for x in (a, b, pa, pb))
+    same = np.all(a == b, axis=1)
+    print("fp16x3 max prob diff", np.abs(pa[:, same] - pb[:, same]).max(), "same", same.mean(), flush=True)
+    assert same.mean() > 0.99
+    assert np.abs(pa[:, same] - pb[:, same]).max() < 1e-5
