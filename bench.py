@@ -97,8 +97,8 @@ def site_hafnian_work_by_site(kres, knum, d, err_tol=1e-10):
 
 def probe_sampling(inst, ko, probe, n, tuning):
     """Time one site's sampling step at full size: the contraction of n carried vectors with
-    a real Gamma_k (FP64, FP32 and BF16x6 forms) and the displacement + draw, then extrapolate
-    the chain's samples/s as n / ((M - 1) x per-site time)."""
+    a real Gamma_k (FP64, FP32, BF16x6 and FP16x3 forms) and the displacement + draw, then
+    extrapolate the chain's samples/s as n / ((M - 1) x per-site time)."""
     import torch
     from bosonsupremacy_amd import runtime, sampling
     ctx = runtime.context()
@@ -142,7 +142,6 @@ def probe_sampling(inst, ko, probe, n, tuning):
     # split once per step (timed with the contraction)
     planes = torch.empty((ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.bfloat16, device=dev)
     ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G32), runtime.ptr(lam_prev), runtime.ptr(planes))
-    del G32
     rows = torch.empty((ctx.lib.bsmps_bf16x6_row_plane_bytes(n, chi_p) // 2,), dtype=torch.bfloat16, device=dev)
 
     def x6():
@@ -150,6 +149,21 @@ def probe_sampling(inst, ko, probe, n, tuning):
         ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(planes),
                  runtime.ptr(T32))
     t6 = timed(x6)
+    del planes, rows
+    # FP16x3: the same with two scaled fp16 parts
+    planes = torch.empty((ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.float16, device=dev)
+    colscale = torch.empty(((chi * d + 63) // 64 * 64,), dtype=torch.float32, device=dev)
+    ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(G32), runtime.ptr(lam_prev), runtime.ptr(planes),
+             runtime.ptr(colscale))
+    del G32
+    rows = torch.empty((ctx.lib.bsmps_fp16x3_row_plane_bytes(n, chi_p) // 2,), dtype=torch.float16, device=dev)
+    rowscale = torch.empty((n,), dtype=torch.float32, device=dev)
+
+    def h3():
+        ctx.call("bsmps_split_rows_fp16x3", n, chi_p, runtime.ptr(pre32), runtime.ptr(rows), runtime.ptr(rowscale))
+        ctx.call("bsmps_site_contract_fp16x3", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(rowscale),
+                 runtime.ptr(planes), runtime.ptr(colscale), runtime.ptr(T32))
+    t3 = timed(h3)
     del planes, rows
     W, x = sampling.displacement_basis(inst.dd)
     Wd, xd = runtime.to_device(W), runtime.to_device(x)
@@ -167,13 +181,16 @@ def probe_sampling(inst, ko, probe, n, tuning):
     f32 = (6.0 if forms["contract_fp32"] == "gauss" else 8.0) * n * chi_p * chi * d
     useful = 8.0 * n * chi_p * chi * d  # complex-GEMM flops
     out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_fp64_s": t64, "contract_fp32_s": t32,
-           "contract_bf16x6_s": t6, "sample_site_s": ts, "contract_forms": forms,
+           "contract_bf16x6_s": t6, "contract_fp16x3_s": t3, "sample_site_s": ts, "contract_forms": forms,
            "contract_fp64_tflops": f64 / t64 / 1e12, "contract_fp32_tflops": f32 / t32 / 1e12,
            "contract_bf16x6_complex_gemm_tflops": useful / t6 / 1e12,
            "contract_bf16x6_mfma_utilisation": 6 * useful / t6 / 1e12 / PEAK_BF16_MFMA_TFLOPS,
+           "contract_fp16x3_complex_gemm_tflops": useful / t3 / 1e12,
+           "contract_fp16x3_mfma_utilisation": 3 * useful / t3 / 1e12 / PEAK_BF16_MFMA_TFLOPS,
            "chain_samples_per_s_fp64": n / ((inst.M - 1) * (t64 + ts)),
            "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts)),
-           "chain_samples_per_s_bf16x6": n / ((inst.M - 1) * (t6 + ts))}
+           "chain_samples_per_s_bf16x6": n / ((inst.M - 1) * (t6 + ts)),
+           "chain_samples_per_s_fp16x3": n / ((inst.M - 1) * (t3 + ts))}
     log(f"sampling probe at site {k}: {out}")
     return out
 

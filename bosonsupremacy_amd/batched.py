@@ -8,7 +8,7 @@ numbers for a batch are generated into the graph's static input buffers before e
 Launch shapes come from the project's autotuner (``autotune.load_or_tune``).
 
 ``run_batch`` times, per instance, the construction and one graph-replayed sampling pass in
-the FP64 form and in each reduced-precision form (FP32, BF16x6 by default; instances/s for
+the FP64 form and in each reduced-precision form (FP32, BF16x6, FP16x3 by default; instances/s for
 each), then measures every reduced form's error against FP64 on the same sample paths: its
 pass is replayed with each uniform moved to the middle of the FP64 cumulative-probability
 interval of the FP64 draw, so both passes follow the FP64 samples (unless the two cdfs differ
@@ -69,7 +69,7 @@ def _mid_uniforms(samples, probs):
     return (cdf.gather(2, idx) - 0.5 * probs.gather(2, idx)).squeeze(2).contiguous()
 
 
-def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6"), verbose=False):
+def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), verbose=False):
     """Construct and sample every instance; returns a summary dict (progress lines go to
     stderr when ``verbose``).  Each form in ``forms`` is timed and checked against FP64."""
     from ._lib import BsmpsError

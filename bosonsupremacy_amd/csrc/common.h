@@ -64,11 +64,13 @@ struct bsmps_ctx {
     size_t hws_bytes;
     size_t split_chunk_bytes;  // upper bound of `hws` per chunk of hafnians
     int contract_gauss[2];     // site contraction form per precision (0 FP64, 1 FP32): 1 = three real products
-    // two pinned host staging buffers for small per-call tables, used alternately; an event
-    // marks when the copy out of each has completed
-    void* pin[2];
-    size_t pin_bytes[2];
-    hipEvent_t pin_ev[2];
+    // a ring of pinned host staging buffers for small per-call tables; an event marks when the
+    // copy out of each has completed, and a buffer is reused only after that (so the host can
+    // run up to kPinSlots uploads ahead of the stream)
+    static constexpr int kPinSlots = 16;
+    void* pin[kPinSlots];
+    size_t pin_bytes[kPinSlots];
+    hipEvent_t pin_ev[kPinSlots];
     int pin_next;
 };
 

@@ -55,8 +55,8 @@ int bsmps_ctx_scratch(bsmps_ctx* ctx, size_t bytes, void** out) {
 int bsmps_ctx_upload(bsmps_ctx* ctx, const void* src, size_t bytes, void* dst) {
     if (bytes == 0) return BSMPS_OK;
     const int i = ctx->pin_next;
-    ctx->pin_next ^= 1;
-    if (ctx->pin_ev[i]) BSMPS_HIP(hipEventSynchronize(ctx->pin_ev[i]));  // the copy two calls ago
+    ctx->pin_next = (ctx->pin_next + 1) % bsmps_ctx::kPinSlots;
+    if (ctx->pin_ev[i]) BSMPS_HIP(hipEventSynchronize(ctx->pin_ev[i]));  // the copy kPinSlots uploads ago
     else BSMPS_HIP(hipEventCreateWithFlags(&ctx->pin_ev[i], hipEventDisableTiming));
     if (bytes > ctx->pin_bytes[i]) {
         if (ctx->pin[i]) BSMPS_HIP(hipHostFree(ctx->pin[i]));
@@ -101,7 +101,7 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     c->split_chunk_bytes = (size_t)4 << 30;
     c->contract_gauss[0] = 0;
     c->contract_gauss[1] = 0;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < bsmps_ctx::kPinSlots; ++i) {
         c->pin[i] = nullptr;
         c->pin_bytes[i] = 0;
         c->pin_ev[i] = nullptr;
@@ -127,7 +127,7 @@ extern "C" int bsmps_destroy(bsmps_ctx* ctx) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) st = BSMPS_ERR_HIP;
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->hws) hipFree(ctx->hws);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < bsmps_ctx::kPinSlots; ++i) {
         if (ctx->pin[i]) hipHostFree(ctx->pin[i]);
         if (ctx->pin_ev[i]) hipEventDestroy(ctx->pin_ev[i]);
     }

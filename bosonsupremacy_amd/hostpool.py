@@ -73,6 +73,17 @@ class _Task:
     def __init__(self, fut, fn, *args):
         self._fut, self._fn, self._args = fut, fn, args
 
+    def done(self):
+        return self._fut is None or isinstance(self._fut, _Done) or self._fut.done()
+
+    def result_or_run(self):
+        """The result if the worker has finished it, else computed here (the worker's copy is
+        dropped), so a caller never waits behind the pool's queue."""
+        if self._fut is not None and not isinstance(self._fut, _Done) and not self._fut.done():
+            self._fut.cancel()
+            self._fut = _Done(self._fn(*self._args))
+        return self.result()
+
     def result(self):
         if self._fut is None:
             self._fut = _Done(self._fn(*self._args))
@@ -106,6 +117,17 @@ def _site_task(k, M, sq_cov, S_prev, S_cur, S_full):
     return gaussian.site_kernel(k, M, sq_cov, cuts, S_full)
 
 
+def _tables_task(k, M, d, chi, res, num_k, num_prev, err_tol, ranks):
+    from . import mps
+    kres = {k: res} if res is not None else {}
+    knum = {}
+    if num_k is not None:
+        knum[k] = num_k
+    if num_prev is not None:
+        knum[k - 1] = num_prev
+    return mps.site_tables(k, M, d, chi, kres, knum, err_tol, ranks)
+
+
 def _williamson_S(V):
     return gaussian.williamson(V)[1]
 
@@ -132,6 +154,23 @@ def cut_inputs_all(sq_cov, cutoff):
 def williamson_S(V):
     """Task (``.result()``) of the Williamson symplectic of V."""
     return _submit(_williamson_S, np.ascontiguousarray(V))
+
+
+def submit_tables(M, d, chi, kres, knum, err_tol, site_d=None, site_chi=None):
+    """Tasks of the SiteTables of every site k = 0..M-1 (mps.site_tables; per-site cutoffs and
+    per-cut ranks when given)."""
+    tasks = []
+    for k in range(M):
+        ranks = None
+        dk = d
+        if site_d is not None or site_chi is not None:
+            ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
+            dk = site_d[k]
+        res = kres[k] if k < M - 1 else None
+        num_k = knum[k] if k < M - 1 else None
+        num_prev = knum[k - 1] if k > 0 else None
+        tasks.append(_submit(_tables_task, k, M, dk, chi, res, num_k, num_prev, err_tol, ranks))
+    return tasks
 
 
 def submit_sites(sq_cov, S_cuts, S_full):

@@ -143,19 +143,64 @@ class _Labels:
         return self
 
 
-def _dev_i32(a):
+class SiteTables:
+    """Host tables of one site tensor: kind, extents, Lambda and the per-label tables of both
+    sides.  Built by ``site_tables`` with NumPy only, so a host worker process can build them."""
+
+    __slots__ = ("k", "kind", "n_a", "n_b", "Lambda", "lam", "left", "right", "sqrt_fact")
+
+
+def site_tables(k, M, d, chi, kron_res, kron_num, err_tol=ERR_TOL, ranks=None):
+    """SiteTables of site k (MPS_cpu.py:402-520: which labels of cuts k-1 and k enter, their
+    targets and denominators, Lambda_k).  ``ranks`` = (rank of cut k-1, rank of cut k) keeps
+    only the leading Schmidt vectors (learned-cutoff form)."""
+    r_prev, r_cur = ranks if ranks is not None else (None, None)
+    t = SiteTables()
+    t.k = k
+    if k == M - 1:
+        t.kind = _lib.SITE_LAST
+        t.left = _Labels.empty()
+        t.lam = np.ones(1)
+        t.n_a = chi if ranks is None else 1
+        t.Lambda = np.zeros(t.n_a)
+    else:
+        t.kind = _lib.SITE_FIRST if k == 0 else _lib.SITE_MIDDLE
+        res = np.asarray(kron_res[k], dtype=np.float64)
+        keep = res > err_tol
+        num_k = np.asarray(kron_num[k]).reshape(len(res), -1)[keep]
+        lam = np.sqrt(res[keep])
+        if r_cur is not None:
+            num_k, lam = num_k[:r_cur], lam[:r_cur]
+        t.left = _Labels(num_k, 0)
+        t.lam = lam
+        t.n_a = chi if ranks is None else len(lam)
+        t.Lambda = np.zeros(t.n_a)
+        t.Lambda[:len(lam)] = lam
+    if k == 0:
+        t.right = _Labels.empty()
+    else:
+        num_pre = np.asarray(kron_num[k - 1])
+        num_pre = num_pre.reshape(num_pre.shape[0], -1)
+        if r_prev is not None:
+            num_pre = num_pre[:r_prev]
+        t.right = _Labels(num_pre, M - k - 1)  # Sigma of site k >= 1 has 2 (M - k) modes
+    t.n_b = chi if ranks is None else len(t.right.count)
+    t.sqrt_fact = np.array([np.sqrt(factorial(j)) for j in np.arange(d)], dtype=np.float64)
+    return t
+
+
+def _i32(a):
     a = np.ascontiguousarray(a, dtype=np.int32)
-    if a.size == 0:
-        a = np.zeros(1, np.int32)
-    return runtime.upload(a)
+    return a if a.size else np.zeros(1, np.int32)
 
 
 def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None,
-                sigma_z=None, ranks=None):
+                sigma_z=None, ranks=None, tables=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
 
     Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site.
-    ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel).
+    ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel);
+    ``tables`` = its SiteTables when already built (site_tables).
 
     ``ranks`` = (rank of cut k-1, rank of cut k) truncates the bond labels to the leading
     Schmidt vectors (the learned-cutoff form; ``d`` is then this site's own cutoff): Gamma is
@@ -164,47 +209,20 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
     ctx = runtime.context()
     dev = ctx.device
     Sigma, Z = sigma_z if sigma_z is not None else gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
+    t = tables if tables is not None else site_tables(k, M, d, chi, kron_res, kron_num, err_tol, ranks)
     Mp = Sigma.shape[0] // 2
-    r_prev, r_cur = ranks if ranks is not None else (None, None)
-    if k == M - 1:
-        kind = _lib.SITE_LAST
-        left = _Labels.empty()
-        lam = np.ones(1)
-        n_a = chi if ranks is None else 1
-        Lambda = np.zeros(n_a)
-    else:
-        kind = _lib.SITE_FIRST if k == 0 else _lib.SITE_MIDDLE
-        res = np.asarray(kron_res[k], dtype=np.float64)
-        keep = res > err_tol
-        num_k = np.asarray(kron_num[k]).reshape(len(res), -1)[keep]
-        lam = np.sqrt(res[keep])
-        if r_cur is not None:
-            num_k, lam = num_k[:r_cur], lam[:r_cur]
-        left = _Labels(num_k, 0)
-        n_a = chi if ranks is None else len(lam)
-        Lambda = np.zeros(n_a)
-        Lambda[:len(lam)] = lam
-    if k == 0:
-        right = _Labels.empty()
-    else:
-        num_pre = np.asarray(kron_num[k - 1])
-        num_pre = num_pre.reshape(num_pre.shape[0], -1)
-        if r_prev is not None:
-            num_pre = num_pre[:r_prev]
-        right = _Labels(num_pre, Mp - 1)
-    n_b = chi if ranks is None else len(right.count)
-    sqrt_fact = np.array([np.sqrt(factorial(j)) for j in np.arange(d)], dtype=np.float64)
-
-    # uploads are asynchronous, so this host work overlaps the previous site's kernels
-    sig_d = runtime.upload(Sigma)
-    sf_d = runtime.upload(sqrt_fact)
-    lt_d, lp_d = _dev_i32(left.targets), _dev_i32(left.perm)
-    rt_d, rp_d = _dev_i32(right.targets), _dev_i32(right.perm)
-    lden_d, rden_d, lam_d = runtime.upload(left.den), runtime.upload(right.den), runtime.upload(lam)
+    if k > 0 and Mp != M - k:
+        raise ValueError(f"site {k}: Sigma has {Mp} modes, expected {M - k}")
+    left, right = t.left, t.right
+    # every table of the site in one pinned staging buffer and one asynchronous copy, which
+    # overlaps the previous site's kernels
+    (sig_d, sf_d, lt_d, lp_d, rt_d, rp_d, lden_d, rden_d, lam_d) = runtime.upload_many(
+        [np.ascontiguousarray(Sigma, dtype=np.complex128), t.sqrt_fact, _i32(left.targets), _i32(left.perm),
+         _i32(right.targets), _i32(right.perm), left.den, right.den, t.lam])
     lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
     rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
     args = _lib.GammaSiteArgs(
-        kind=kind, d=d, chi=n_a, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
+        kind=t.kind, d=d, chi=t.n_a, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
         sqrt_fact=sf_d.data_ptr(),
         n_left=len(left.count), w_left=left.targets.shape[1], left_targets=lt_d.data_ptr(),
         left_den=lden_d.data_ptr(), lambda_=lam_d.data_ptr(), left_perm=lp_d.data_ptr(),
@@ -212,10 +230,10 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
         n_right=len(right.count), w_right=right.targets.shape[1], right_targets=rt_d.data_ptr(),
         right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
         right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
-    gamma = torch.zeros((n_b, n_a, d), dtype=torch.complex128, device=dev)
+    gamma = torch.zeros((t.n_b, t.n_a, d), dtype=torch.complex128, device=dev)
     with (timer("gamma_site") if timer else contextlib.nullcontext()):
         ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
-    return gamma, Lambda
+    return gamma, t.Lambda
 
 
 class HostStream:
@@ -286,17 +304,23 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
             site_chi = [int((np.asarray(r) > ERR_TOL).sum()) for r in kres]
     streamer = HostStream() if to_host else None
     wanted = set(range(M)) if sites is None else set(sites)
+    # the label tables of every site are built by the host workers while the GPU runs the
+    # previous sites (a site whose tables are not ready yet builds them here instead)
+    tab_f = hostpool.submit_tables(M, d, chi, kres, knum, ERR_TOL, site_d if learned else None,
+                                   site_chi if learned else None) if sites is None else None
     for k in range(M):
         if k not in wanted:
             Gammas.append(None)
             Lambdas.append(None)
             continue
+        tables = tab_f[k].result_or_run() if tab_f is not None else None
         if learned:
             ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, site_d[k], chi, timer=timer,
-                               sigma_z=site_f[k].result(), ranks=ranks)
+                               sigma_z=site_f[k].result(), ranks=ranks, tables=tables)
         else:
-            G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result())
+            G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result(),
+                               tables=tables)
         if streamer is not None:
             G = streamer.push(G)
         Gammas.append(G)

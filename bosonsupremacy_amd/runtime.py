@@ -129,5 +129,30 @@ def upload(a, device=None):
     return torch.from_numpy(arr).pin_memory().to(device, non_blocking=True)
 
 
+_TORCH_DTYPES = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                 np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64, np.dtype(np.int8): torch.int8,
+                 np.dtype(np.uint8): torch.uint8, np.dtype(np.complex128): torch.complex128,
+                 np.dtype(np.complex64): torch.complex64}
+
+
+def upload_many(arrays, device=None):
+    """Several numpy arrays -> device tensors through one pinned staging buffer and one
+    asynchronous copy on the current stream (each array 256-byte aligned in it); the tensors
+    are views of one device allocation."""
+    device = device or context().device
+    arrs = [np.ascontiguousarray(a) for a in arrays]
+    offs, total = [], 0
+    for a in arrs:
+        total = (total + 255) // 256 * 256
+        offs.append(total)
+        total += a.nbytes
+    host = torch.empty((max(total, 1),), dtype=torch.uint8, pin_memory=True)
+    hv = host.numpy()
+    for a, o in zip(arrs, offs):
+        hv[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+    dev = host.to(device, non_blocking=True)
+    return [dev[o:o + a.nbytes].view(_TORCH_DTYPES[a.dtype]).view(a.shape) for a, o in zip(arrs, offs)]
+
+
 def to_host(t):
     return t.detach().cpu().numpy()

@@ -27,7 +27,7 @@ def _replays_match(batched, instances, mps, sampling):
     G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     n = 700
-    for prec in ("fp64", "fp32", "bf16x6"):
+    for prec in ("fp64", "fp32", "bf16x6", "fp16x3"):
         eager = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
         gs = batched.GraphSampler(G, L, sqrtW, inst.dd, n, prec, return_probs=True)
         for rep in range(3):
@@ -53,7 +53,7 @@ def test_small_random_batch_runs():
     out = batched.run_batch(insts, n_samples=500)
     print({k: v for k, v in out.items() if k != "per_instance"}, flush=True)
     assert out["instances"] + len(out["skipped"]) == 3 and out["instances"] >= 1
-    for f in ("fp32", "bf16x6"):
+    for f in ("fp32", "bf16x6", "fp16x3"):
         assert out[f"{f}_marginal_abs_err_max"] < 1e-6
         assert out[f"{f}_off_fp64_path_total"] == 0
         assert out[f"instances_per_s_{f}"] > 0

@@ -28,3 +28,33 @@ def test_single_thread_blas_is_reproducible():
     with threadpool_limits(4):
         S4 = gaussian.cut_inputs(inst.sq_cov, 1, inst.d)[1]
     assert np.array_equal(S, S4)
+
+
+def test_site_tables_from_pool_match_serial_and_targets(golden_m8):
+    """Label tables built by the host workers equal those built in-process, and their targets
+    are the reference's get_target rows (right side shifted past the site's output modes)."""
+    from bosonsupremacy_amd import mps
+    g = golden_m8
+    M, d, chi = 8, int(g["d"]), int(g["chi"])
+    kres = [g[f"kron_res16_{k}"].astype(np.float64) for k in range(M - 1)]
+    knum = [g[f"kron_num_{k}"] for k in range(M - 1)]
+    pooled = [t.result() for t in hostpool.submit_tables(M, d, chi, kres, knum, 1e-10)]
+    for k in range(M):
+        t = mps.site_tables(k, M, d, chi, kres, knum)
+        p = pooled[k]
+        assert (t.kind, t.n_a, t.n_b) == (p.kind, p.n_a, p.n_b) and np.array_equal(t.Lambda, p.Lambda)
+        for a, b in ((t.left, p.left), (t.right, p.right)):
+            for name in ("targets", "den", "perm", "class_count", "count"):
+                assert np.array_equal(getattr(a, name), getattr(b, name)), (k, name)
+        def padded(a, w):  # zero padding to the reference's row width
+            out = np.zeros((a.shape[0], max(w, a.shape[1])), np.int32)
+            out[:, :a.shape[1]] = a
+            return out
+        if k < M - 1:
+            keep = kres[k] > 1e-10
+            want = g[f"target_{k}"][keep]
+            assert np.array_equal(padded(t.left.targets, want.shape[1]), padded(want, 0)), k
+        if k > 0:
+            want = g[f"target_{k - 1}"]
+            shifted = np.where(want != 0, want + (M - k - 1), 0)
+            assert np.array_equal(padded(t.right.targets, want.shape[1]), padded(shifted, 0)), k
