@@ -189,18 +189,48 @@ def site_tables(k, M, d, chi, kron_res, kron_num, err_tol=ERR_TOL, ranks=None):
     return t
 
 
+def right_label_runs(t, d, n_runs):
+    """Split the right labels of SiteTables ``t`` (in photon-class order, the order of
+    ``t.right.perm``) into at most ``n_runs`` contiguous runs of about equal hafnian work, as
+    [(first, end)] positions in that order.  The work of one right label of class q is the sum,
+    over photon numbers j < d and left classes p with j + p + q even, of the left class size
+    times m^3 2^m for the hafnian of size 2m = j + p + q."""
+    rcls = t.right.count[t.right.perm]
+    n = len(rcls)
+    if n_runs <= 1 or n <= 1:
+        return [(0, n)]
+    lcc = np.asarray(t.left.class_count, dtype=np.float64)
+    per_class = np.zeros(len(t.right.class_count))
+    for q in range(len(per_class)):
+        for j in range(d):
+            for p in range(len(lcc)):
+                s = j + p + q
+                if s % 2 == 0 and lcc[p]:
+                    m = s // 2
+                    per_class[q] += lcc[p] * max(1, m ** 3) * 2.0 ** m
+    cum = np.cumsum(per_class[rcls])
+    cuts = np.searchsorted(cum, cum[-1] * np.arange(1, n_runs) / n_runs, side="right")
+    edges = np.unique(np.concatenate([[0], np.clip(cuts, 1, n - 1), [n]]))
+    return [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]
+
+
 def _i32(a):
     a = np.ascontiguousarray(a, dtype=np.int32)
     return a if a.size else np.zeros(1, np.int32)
 
 
 def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None,
-                sigma_z=None, ranks=None, tables=None):
+                sigma_z=None, ranks=None, tables=None, right_runs=1, on_run=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
 
     Follows the three branches of MPS_cpu.py:402-520; Lambda is zero for the last site.
     ``sigma_z`` = (Sigma, Z) of the site when already computed (gaussian.site_kernel);
     ``tables`` = its SiteTables when already built (site_tables).
+
+    ``right_runs`` > 1 computes the site in that many library calls, each over a run of the
+    right labels (right_label_runs), and calls ``on_run(i, n_runs)`` after enqueuing run i --
+    for progress reports on sites that take minutes.  Every entry is the same hafnian either
+    way, so the result is bit-identical.
 
     ``ranks`` = (rank of cut k-1, rank of cut k) truncates the bond labels to the leading
     Schmidt vectors (the learned-cutoff form; ``d`` is then this site's own cutoff): Gamma is
@@ -231,8 +261,17 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
         right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
         right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
     gamma = torch.zeros((t.n_b, t.n_a, d), dtype=torch.complex128, device=dev)
-    with (timer("gamma_site") if timer else contextlib.nullcontext()):
-        ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
+    runs = right_label_runs(t, d, right_runs)
+    for i, (r0, r1) in enumerate(runs):
+        if len(runs) > 1:  # this run's labels: a slice of the class-ordered permutation
+            rcc = np.bincount(right.count[right.perm[r0:r1]], minlength=len(right.class_count)).astype(np.int32)
+            args.n_right = r1 - r0
+            args.right_perm = rp_d.data_ptr() + 4 * r0
+            args.right_class_count_host = rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        with (timer("gamma_site") if timer else contextlib.nullcontext()):
+            ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
+        if on_run is not None:
+            on_run(i, len(runs))
     return gamma, t.Lambda
 
 
@@ -269,7 +308,7 @@ class HostStream:
 
 
 def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=None, site_chi=None,
-              to_host=False, sites=None):
+              to_host=False, sites=None, right_runs=None, on_run=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
     outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them; when it is
     None the spectra are computed too.
@@ -279,6 +318,8 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
     ``to_host`` streams every Gamma_k to pinned host memory as soon as it is computed (the
     returned Gammas are then pinned host tensors and the device copies are released);
     ``sites`` builds only the listed sites (None entries elsewhere).
+    ``right_runs`` = {site: number of runs} computes those sites in runs of right labels,
+    calling ``on_run(site, i, n_runs)`` after each (site_tensor).
 
     Pipelined: the host FP64 work of every site (Bloch-Messiah, Sigma) is queued on the
     host worker pool as soon as the cut symplectics are known -- before the Schmidt spectra
@@ -314,13 +355,17 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
             Lambdas.append(None)
             continue
         tables = tab_f[k].result_or_run() if tab_f is not None else None
+        runs = {}
+        if right_runs and right_runs.get(k, 1) > 1:
+            runs = {"right_runs": right_runs[k],
+                    "on_run": (lambda i, n, k=k: on_run(k, i, n)) if on_run is not None else None}
         if learned:
             ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, site_d[k], chi, timer=timer,
-                               sigma_z=site_f[k].result(), ranks=ranks, tables=tables)
+                               sigma_z=site_f[k].result(), ranks=ranks, tables=tables, **runs)
         else:
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, timer=timer, sigma_z=site_f[k].result(),
-                               tables=tables)
+                               tables=tables, **runs)
         if streamer is not None:
             G = streamer.push(G)
         Gammas.append(G)

@@ -113,3 +113,21 @@ def test_sites_m32_sampled_entries_match_oracle():
         # odd-size entries stay exactly zero
         odd = (la[:, None] + lb[None, :]) % 2 == 1
         assert np.all(Gh[:nR, :nL, 0].T[odd] == 0)
+
+
+def test_site_in_right_label_runs_is_bit_identical():
+    """A site computed in several calls over runs of its right labels equals the one-call site."""
+    from bosonsupremacy_amd import instances, kron, mps, runtime, gaussian
+    inst = instances.workload("m32_d6_chi512")
+    kres, knum, kS = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+    S_full = gaussian.williamson(inst.sq_cov)[1]
+    for k in (0, 9, 31):
+        G1, L1 = mps.site_tensor(k, inst.M, inst.sq_cov, kres, knum, kS, inst.d, inst.chi, S_full)
+        seen = []
+        G5, L5 = mps.site_tensor(k, inst.M, inst.sq_cov, kres, knum, kS, inst.d, inst.chi, S_full,
+                                 right_runs=5, on_run=lambda i, n: seen.append((i, n)))
+        print(f"site {k}: {len(seen)} runs", flush=True)
+        assert same_bits(runtime.to_host(G5), runtime.to_host(G1)) and same_bits(L5, L1), k
+        # at most the runs asked for (labels of equal work are not split), one call per run
+        assert 1 <= len(seen) <= 5 and [i for i, _ in seen] == list(range(seen[-1][1])), k
+        assert k != 9 or len(seen) == 5

@@ -58,3 +58,33 @@ def test_site_tables_from_pool_match_serial_and_targets(golden_m8):
             want = g[f"target_{k - 1}"]
             shifted = np.where(want != 0, want + (M - k - 1), 0)
             assert np.array_equal(padded(t.right.targets, want.shape[1]), padded(shifted, 0)), k
+
+
+def test_right_label_runs_partition_the_labels(golden_m8):
+    """Runs of right labels are contiguous, cover every label once, respect the requested count
+    and balance the m^3 2^m work estimate."""
+    from bosonsupremacy_amd import mps
+    g = golden_m8
+    M, d, chi = 8, int(g["d"]), int(g["chi"])
+    kres = [g[f"kron_res16_{k}"].astype(np.float64) for k in range(M - 1)]
+    knum = [g[f"kron_num_{k}"] for k in range(M - 1)]
+    for k in range(M):
+        t = mps.site_tables(k, M, d, chi, kres, knum)
+        n = len(t.right.perm)
+        assert mps.right_label_runs(t, d, 1) == [(0, n)]
+        for want in (2, 3, 7, 1000):
+            runs = mps.right_label_runs(t, d, want)
+            assert runs[0][0] == 0 and runs[-1][1] == n and len(runs) <= min(want, n)
+            assert all(a < b for a, b in runs) and all(runs[i][1] == runs[i + 1][0] for i in range(len(runs) - 1))
+            # per-run class counts add up to the site's
+            tot = sum(np.bincount(t.right.count[t.right.perm[a:b]], minlength=len(t.right.class_count))
+                      for a, b in runs)
+            assert np.array_equal(tot, t.right.class_count), (k, want)
+        if n >= 16:  # two runs: each within one label's work of half the total
+            lcc = t.left.class_count
+            def w(q):
+                return sum(lcc[p] * max(1, ((j + p + q) // 2) ** 3) * 2.0 ** ((j + p + q) // 2)
+                           for j in range(d) for p in range(len(lcc)) if (j + p + q) % 2 == 0 and lcc[p])
+            per = np.array([w(q) for q in t.right.count[t.right.perm]])
+            (a0, b0), (a1, b1) = mps.right_label_runs(t, d, 2)
+            assert abs(per[a0:b0].sum() - per[a1:b1].sum()) <= 2 * per.max() + 1e-9 * per.sum(), k
