@@ -897,10 +897,12 @@ __global__ __launch_bounds__(256) void site_displace_kernel(int64_t n, int d, in
 // tightest of a few bounds is launched: fewer registers, more waves per SIMD).
 // One wave per sample, four samples per workgroup, each wave independent; D(alpha_s) comes from
 // site_displace_kernel.  TT: storage type of T and pre (double: FP64 form; float:
-// reduced-precision forms, where the carried vector is rescaled by its largest component after
-// every site so that a long chain cannot underflow -- the reference's GPU path does the same,
-// distributed_sampling.py:147).  The probabilities and the inverse CDF are evaluated in FP64 in
-// every form.
+// reduced-precision forms, where the carried vector is rescaled after every site so that a long
+// chain cannot underflow -- by 1/sqrt(p_pick), which makes its lam-weighted norm 1; the
+// reference's GPU path rescales by the largest component, distributed_sampling.py:147, and the
+// draws only see ratios of probabilities either way).  The probabilities and the inverse CDF are
+// evaluated in FP64 in every form.  Used for d > 8 or dd > 16 (sample_site_gram_kernel below
+// covers the smaller cutoffs).
 template <int DMAX, int DDMAX, typename TT, bool RENORM>
 __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
                                                           const TT* __restrict__ T, int64_t tstride,
@@ -923,14 +925,10 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
         }
         wave_lds_sync();
         const TT* Ts = T + 2 * se * tstride;
-        // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda);
-        // with RENORM also mx[k] = max_a max(|Re|, |Im|) of the candidate row sum_j D[k][j] T[a][j],
-        // so the rescaled carried vector needs one more pass over T, not two
-        double pk[DDMAX], mx[RENORM ? DDMAX : 1];
+        // p[k] = sum_a |sum_j D[k][j] T[a][j]|^2 lam[a]^2   (last site: a = 0 only, no lambda)
+        double pk[DDMAX];
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) pk[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < (RENORM ? DDMAX : 1); ++k) mx[k] = 0.0;
         const int a_end = lam ? chi : 1;
         for (int aa = lane; aa < a_end; aa += 64) {
             cd t[DMAX];
@@ -950,25 +948,20 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                         }
                     }
                     pk[k] = fma(fma(ur, ur, ui * ui), l2, pk[k]);
-                    if constexpr (RENORM) mx[k] = fmax(mx[k], fmax(fabs(ur), fabs(ui)));
                 }
             }
         }
-        // wave reduction; lane 0 ends with the totals (every lane with the maxima)
+        // wave reduction; lane 0 ends with the totals
 #pragma unroll
         for (int k = 0; k < DDMAX; ++k) {
             if (k < dd) {
                 double v = pk[k];
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
                 pk[k] = v;
-                if constexpr (RENORM) {
-                    double m = mx[k];
-                    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-                    mx[k] = m;
-                }
             }
         }
         int pick = 0;
+        double scale = 1.0;
         if (lane == 0) {
             double tot = 0.0;
 #pragma unroll
@@ -988,8 +981,14 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
             }
             pick = cntk < dd - 1 ? cntk : dd - 1;
             if (live) samples[s * sstride] = pick;
+            double pp = 0.0;
+#pragma unroll
+            for (int k = 0; k < DDMAX; ++k)
+                if (k == pick) pp = pk[k];
+            scale = pp > 0.0 ? 1.0 / sqrt(pp) : 1.0;
         }
         pick = __shfl(pick, 0, 64);
+        if constexpr (RENORM) scale = __shfl(scale, 0, 64);
         if (pre && live) {
             auto sel = [&](int aa) -> cd {
                 double ur = 0.0, ui = 0.0;
@@ -1001,14 +1000,6 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
                 }
                 return cd{ur, ui};
             };
-            double scale = 1.0;
-            if constexpr (RENORM) {
-                double m = 0.0;
-#pragma unroll
-                for (int k = 0; k < DDMAX; ++k)
-                    if (k == pick) m = mx[k];
-                scale = m > 0.0 ? 1.0 / m : 1.0;
-            }
             for (int aa = lane; aa < chi; aa += 64) {
                 cd u = sel(aa);
                 if constexpr (RENORM) u = cscale(u, scale);
@@ -1017,6 +1008,250 @@ __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, in
         }
         wave_lds_sync();
     }
+}
+
+// One DPP step of a 16-lane butterfly on a double (two 32-bit moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a DPP row; every lane of the row ends with the same total (each
+// step adds a pair of equal partial sums in either order).
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+// Probabilities + draw + carried vector through the per-sample Gram matrix, for d <= 8 and
+// dd <= 16: one workgroup of 256 threads per sample.  The sample's rows of T are staged through
+// LDS in chunks of R rows with coalesced loads (consecutive threads read consecutive elements;
+// row a's element j lands at a * RS + j, RS = d rounded up to odd against bank conflicts), and
+// when all chi rows fit (R >= chi) the carried-vector pass reads them from LDS again instead of
+// from HBM.  Pass 1 accumulates G = sum_a lam_a^2 t_a t_a^H (d(d+1)/2 complex entries in VGPRs,
+// one row per thread per step); a DPP butterfly sums each 16-lane row, and the 16 row partials
+// are added in a fixed order through LDS.  Thread k then forms p_k = D_k G D_k^H -- the same
+// conditional probability as sum_a lam_a^2 |D_k t_a|^2 at (d+1)/(2 dd) of the FP64 work -- and
+// thread 0 draws (sequential total and cdf, as in sample_site_kernel).  Pass 2 writes the carried
+// vector pre[a] = D_pick t_a with the same operation order as sample_site_kernel, so the FP64
+// form's carried vectors are the same bits.  RENORM scales the carried vector by 1/sqrt(p_pick),
+// which makes its lam-weighted norm 1 at every site (the draws only see ratios of probabilities).
+template <typename TT>
+struct GramStage;
+template <>
+struct GramStage<double> {
+    static constexpr int kBytes = 64 * 1024;  // two workgroups per CU (160 KB of LDS)
+    using V = double2;
+};
+template <>
+struct GramStage<float> {
+    static constexpr int kBytes = 40 * 1024;  // three workgroups per CU with the other arrays
+    using V = float2;
+};
+
+template <int DMAX, int DDMAX, typename TT, bool RENORM>
+__global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int chi, int d, int dd,
+                                                               const TT* __restrict__ T, int64_t tstride,
+                                                               const double* __restrict__ Dg,
+                                                               const double* __restrict__ lam,
+                                                               const double* __restrict__ uni,
+                                                               int32_t* __restrict__ samples, int64_t sstride,
+                                                               TT* __restrict__ pre, double* __restrict__ probs,
+                                                               int R, int RS, unsigned dmagic) {
+    static_assert(DDMAX <= 64 && DMAX <= DDMAX, "one candidate photon number per thread");
+    using V = typename GramStage<TT>::V;
+    constexpr int NP = DMAX * (DMAX + 1) / 2;
+    __shared__ V s_T[GramStage<TT>::kBytes / sizeof(V)];
+    __shared__ double s_part[16][2 * NP];  // per 16-lane row: G partial (re, im per entry)
+    __shared__ double s_G[2 * NP];
+    __shared__ double s_p[DDMAX];
+    __shared__ double s_scale;
+    __shared__ int s_pick;
+    const int tid = threadIdx.x, r16 = tid % 16, row = tid / 16;
+    // rows [r0, r0 + nr) of the sample into s_T, four loads in flight per thread (more loads
+    // per batch were measured slower); the caller synchronises around it
+    auto stage = [&](const V* Tv, int r0, int nr) {
+        const int ne = nr * d;
+        const V* src = Tv + (int64_t)r0 * d;
+        int e = tid;
+        for (; e + 3 * 256 < ne; e += 4 * 256) {
+            V v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = src[e + u * 256];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int ee = e + u * 256;
+                const int a = (int)__umulhi((unsigned)ee, dmagic);
+                s_T[a * RS + (ee - a * d)] = v[u];
+            }
+        }
+        for (; e < ne; e += 256) {
+            const int a = (int)__umulhi((unsigned)e, dmagic);
+            s_T[a * RS + (e - a * d)] = src[e];
+        }
+    };
+    auto row_of = [&](int a, cd* t) {
+#pragma unroll
+        for (int j = 0; j < DMAX; ++j) {
+            if (j < d) {
+                const V v = s_T[a * RS + j];
+                t[j] = cd{(double)v.x, (double)v.y};
+            } else {
+                t[j] = czero();
+            }
+        }
+    };
+    for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+        const V* Tv = reinterpret_cast<const V*>(T) + s * tstride;
+        double gr[NP], gi[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) gr[p] = gi[p] = 0.0;
+        const int a_end = lam ? chi : 1;
+        for (int r0 = 0; r0 < a_end; r0 += R) {
+            const int nr = a_end - r0 < R ? a_end - r0 : R;
+            __syncthreads();  // the previous chunk / sample is no longer read
+            stage(Tv, r0, nr);
+            __syncthreads();
+            for (int aa = tid; aa < nr; aa += 256) {
+                const double l = lam ? lam[r0 + aa] : 1.0;
+                cd t[DMAX];
+                row_of(aa, t);
+                int p = 0;
+#pragma unroll
+                for (int j = 0; j < DMAX; ++j) {
+#pragma unroll
+                    for (int q = j; q < DMAX; ++q, ++p) {
+                        if (q < d) {
+                            const cd x = cscale(t[j], l), y = cscale(t[q], l);
+                            if (q == j) {
+                                gr[p] = fma(x.re, x.re, fma(x.im, x.im, gr[p]));
+                            } else {  // t_j conj(t_q)
+                                gr[p] = fma(x.re, y.re, fma(x.im, y.im, gr[p]));
+                                gi[p] = fma(x.im, y.re, fma(-x.re, y.im, gi[p]));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        {
+            int p = 0;
+#pragma unroll
+            for (int j = 0; j < DMAX; ++j) {
+#pragma unroll
+                for (int q = j; q < DMAX; ++q, ++p) {
+                    if (q < d) {
+                        const double vr = row16_sum(gr[p]);
+                        const double vi = q != j ? row16_sum(gi[p]) : 0.0;
+                        if (r16 == 0) {
+                            s_part[row][2 * p] = vr;
+                            s_part[row][2 * p + 1] = vi;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < 2 * NP) {
+            double acc = 0.0;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) acc += s_part[w][tid];
+            s_G[tid] = acc;
+        }
+        __syncthreads();
+        // p_k = sum_j |D_kj|^2 G_jj + 2 Re sum_{j<q} D_kj G_jq conj(D_kq), thread k
+        if (tid < dd) {
+            cd dk[DMAX];
+#pragma unroll
+            for (int j = 0; j < DMAX; ++j) dk[j] = j < d ? ld_cd(Dg, (s * dd + tid) * d + j) : czero();
+            double diag = 0.0, off = 0.0;
+            int p = 0;
+#pragma unroll
+            for (int j = 0; j < DMAX; ++j) {
+#pragma unroll
+                for (int q = j; q < DMAX; ++q, ++p) {
+                    if (q < d) {
+                        const double g_re = s_G[2 * p], g_im = s_G[2 * p + 1];
+                        if (q == j) {
+                            diag = fma(fma(dk[j].re, dk[j].re, dk[j].im * dk[j].im), g_re, diag);
+                        } else {
+                            const double zr = dk[j].re * g_re - dk[j].im * g_im;
+                            const double zi = dk[j].re * g_im + dk[j].im * g_re;
+                            off = fma(zr, dk[q].re, fma(zi, dk[q].im, off));
+                        }
+                    }
+                }
+            }
+            s_p[tid] = fmax(fma(2.0, off, diag), 0.0);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double tot = 0.0;
+            for (int k = 0; k < dd; ++k) tot += s_p[k];
+            const double u = uni[s];
+            double cdf = 0.0;
+            int cntk = 0;
+            for (int k = 0; k < dd; ++k) {
+                const double pnorm = s_p[k] / tot;
+                cdf += pnorm;
+                if (u > cdf) ++cntk;
+                if (probs) probs[s * dd + k] = pnorm;
+            }
+            const int pick = cntk < dd - 1 ? cntk : dd - 1;
+            samples[s * sstride] = pick;
+            const double pp = s_p[pick];
+            s_pick = pick;
+            s_scale = pp > 0.0 ? 1.0 / sqrt(pp) : 1.0;
+        }
+        __syncthreads();
+        if (pre) {
+            const int pick = s_pick;
+            const double scale = s_scale;
+            cd dp[DMAX];
+#pragma unroll
+            for (int j = 0; j < DMAX; ++j) dp[j] = j < d ? ld_cd(Dg, (s * dd + pick) * d + j) : czero();
+            const bool resident = a_end == chi && R >= chi;  // pass 1 left every row in s_T
+            for (int r0 = 0; r0 < chi; r0 += R) {
+                const int nr = chi - r0 < R ? chi - r0 : R;
+                if (!resident) {
+                    __syncthreads();
+                    stage(Tv, r0, nr);
+                    __syncthreads();
+                }
+                for (int aa = tid; aa < nr; aa += 256) {
+                    cd t[DMAX];
+                    row_of(aa, t);
+                    double ur = 0.0, ui = 0.0;
+#pragma unroll
+                    for (int j = 0; j < DMAX; ++j) {
+                        if (j < d) {
+                            ur = fma(dp[j].re, t[j].re, fma(-dp[j].im, t[j].im, ur));
+                            ui = fma(dp[j].re, t[j].im, fma(dp[j].im, t[j].re, ui));
+                        }
+                    }
+                    cd v{ur, ui};
+                    if constexpr (RENORM) v = cscale(v, scale);
+                    st_c(pre, s * chi + r0 + aa, v);
+                }
+            }
+        }
+        // s_T is next written behind the next sample's first barrier; s_part / s_G / s_p behind
+        // its later barriers, s_pick / s_scale behind its third after the staging
+    }
+}
+
+// Rows per staging chunk and the odd row stride of sample_site_gram_kernel.
+template <typename TT>
+void gram_stage_shape(int chi, int d, int& R, int& RS, unsigned& dmagic) {
+    RS = d | 1;
+    const int cap = GramStage<TT>::kBytes / (int)(RS * 2 * sizeof(TT));
+    R = chi < cap ? chi : cap;
+    dmagic = (unsigned)((((uint64_t)1 << 32) + d - 1) / d);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1156,11 +1391,19 @@ extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int 
 #define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
     hipLaunchKernelGGL((sample_site_kernel<DM, DDM, double, false>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
                        T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
-    if (d <= 4 && dd <= 8) BSMPS_SAMPLE_LAUNCH(4, 8);
-    else if (d <= 6 && dd <= 12) BSMPS_SAMPLE_LAUNCH(6, 12);
-    else if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+#define BSMPS_GRAM_LAUNCH(DM, DDM)                                                                              \
+    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, double, false>), grid16, block, 0, ctx->stream, n, chi, d, dd, \
+                       T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS, dmagic)
+    const dim3 grid16(grid_for(n, 1));
+    int R = 0, RS = 0;
+    unsigned dmagic = 0;
+    gram_stage_shape<double>(chi, d, R, RS, dmagic);
+    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8);
+    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12);
+    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
+#undef BSMPS_GRAM_LAUNCH
 #undef BSMPS_SAMPLE_LAUNCH
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
@@ -1236,11 +1479,19 @@ extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, 
 #define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
     hipLaunchKernelGGL((sample_site_kernel<DM, DDM, float, true>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
                        T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
-    if (d <= 4 && dd <= 8) BSMPS_SAMPLE_LAUNCH(4, 8);
-    else if (d <= 6 && dd <= 12) BSMPS_SAMPLE_LAUNCH(6, 12);
-    else if (d <= 8 && dd <= 16) BSMPS_SAMPLE_LAUNCH(8, 16);
+#define BSMPS_GRAM_LAUNCH(DM, DDM)                                                                              \
+    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, float, true>), grid16, block, 0, ctx->stream, n, chi, d, dd, \
+                       T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS, dmagic)
+    const dim3 grid16(grid_for(n, 1));
+    int R = 0, RS = 0;
+    unsigned dmagic = 0;
+    gram_stage_shape<float>(chi, d, R, RS, dmagic);
+    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8);
+    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12);
+    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
+#undef BSMPS_GRAM_LAUNCH
 #undef BSMPS_SAMPLE_LAUNCH
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
