@@ -259,6 +259,8 @@ def partial_run(args):
                                                 inst.d, inst.chi)
     lwork = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_d=site_d, site_chi=site_chi)
     plan.update({"site_cutoffs": [int(x) for x in site_d], "cut_ranks": [int(x) for x in site_chi],
+                 "work_by_site": [{"hafnians": int(w[0]), "flops": float(w[1]), "largest_size": int(w[2])}
+                                  for w in lwork],
                  "total_flops": float(sum(w[1] for w in lwork)),
                  "plain_total_flops": float(sum(w[1] for w in work))})
     log(f"learned plan: mean cutoff {plan['mean_cutoff']:.2f}, mean rank {plan['mean_rank']:.0f}, "
