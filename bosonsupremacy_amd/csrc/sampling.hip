@@ -545,6 +545,15 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // 16-byte group g (of 4) of LDS row r is stored in slot g ^ f(r)
 __device__ __forceinline__ int x6_slot(int r, int g) { return g ^ (((r >> 2) ^ (r >> 1)) & 3); }
 
+// Plane layout of the split forms, for a row (carried vector) or column (Gamma) r: element k
+// of part p at ((r kpad / KC + k / KC) NPL + p) KC + k % KC.  One KC-step of all NPL parts of a
+// row is NPL * 64 contiguous bytes, so a GEMM step reads whole 128-byte lines however long the
+// rows are (with the parts in separate planes each step read half a line per part, and at
+// kpad = 10^4 the other half was evicted before the next step).
+__device__ __forceinline__ int64_t x6_at(int64_t r, int k, int kpad, int npl, int p) {
+    return ((r * (kpad / X6_KC) + k / X6_KC) * npl + p) * X6_KC + k % X6_KC;
+}
+
 // Operand formats of the split-precision contraction.  An FP32 value is split into NP parts
 // of a 16-bit type; prod() adds the retained part products of one real product to acc.
 struct Bf16x6Fmt {  // three bf16 parts (exact), products of order >= 2^-16: six MFMAs
@@ -589,10 +598,10 @@ __device__ __forceinline__ void split2_fp16(float x, _Float16& h, _Float16& l) {
     l = (_Float16)(x - (float)h);
 }
 
-// planes[p][c][k] of lambda[k] * gamma[k][c]; c < ncols_pad, k < kpad (zero padded)
+// parts p of lambda[k] * gamma[k][c] at x6_at(c, k, kpad, 6, p); c < ncols_pad, k < kpad (zero padded)
 __global__ void split_bf16x6_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
                                     int ncols, int kpad, int ncols_pad, __bf16* __restrict__ planes) {
-    const int64_t total = (int64_t)ncols_pad * kpad, plane = total;
+    const int64_t total = (int64_t)ncols_pad * kpad;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(i / kpad), k = (int)(i % kpad);
         float2 g = make_float2(0.f, 0.f);
@@ -602,21 +611,21 @@ __global__ void split_bf16x6_kernel(const float* __restrict__ gamma, const doubl
             g.x = (float)(g.x * l);
             g.y = (float)(g.y * l);
         }
-        split3(g.x, planes[i], planes[plane + i], planes[2 * plane + i]);
-        split3(g.y, planes[3 * plane + i], planes[4 * plane + i], planes[5 * plane + i]);
+        split3(g.x, planes[x6_at(c, k, kpad, 6, 0)], planes[x6_at(c, k, kpad, 6, 1)], planes[x6_at(c, k, kpad, 6, 2)]);
+        split3(g.y, planes[x6_at(c, k, kpad, 6, 3)], planes[x6_at(c, k, kpad, 6, 4)], planes[x6_at(c, k, kpad, 6, 5)]);
     }
 }
 
-// planes[p][r][k] of pre[r][k] (complex64 [n][chi_prev]); k < kpad (zero padded)
+// parts p of pre[r][k] (complex64 [n][chi_prev]) at x6_at(r, k, kpad, 6, p); k < kpad (zero padded)
 __global__ void split_rows_bf16x6_kernel(const float* __restrict__ pre, int64_t n, int chi_prev, int kpad,
                                          __bf16* __restrict__ planes) {
-    const int64_t total = n * kpad, plane = total;
+    const int64_t total = n * kpad;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = i / kpad;
         const int k = (int)(i % kpad);
         const float2 v = k < chi_prev ? reinterpret_cast<const float2*>(pre)[r * chi_prev + k] : make_float2(0.f, 0.f);
-        split3(v.x, planes[i], planes[plane + i], planes[2 * plane + i]);
-        split3(v.y, planes[3 * plane + i], planes[4 * plane + i], planes[5 * plane + i]);
+        split3(v.x, planes[x6_at(r, k, kpad, 6, 0)], planes[x6_at(r, k, kpad, 6, 1)], planes[x6_at(r, k, kpad, 6, 2)]);
+        split3(v.y, planes[x6_at(r, k, kpad, 6, 3)], planes[x6_at(r, k, kpad, 6, 4)], planes[x6_at(r, k, kpad, 6, 5)]);
     }
 }
 
@@ -668,7 +677,7 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
     const int64_t row0 = (int64_t)rb * X6_TM;
     const int col0 = cb * X6_TN;  // complex column
     const int wm = (wave / 2) * 64, wn = (wave % 2) * 32;
-    const int64_t a_plane = n * kpad, b_plane = (int64_t)ncols_pad * kpad;
+    const int64_t ksteps = kpad / X6_KC;
 
     float4_t acc_re[4][2], acc_im[4][2];
 #pragma unroll
@@ -676,34 +685,37 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-    // staging pieces of 16 bytes: A NPL planes x 128 rows x 4 groups (2 NPL per thread, fetched into
-    // registers one chunk ahead), B NPL planes x 64 columns x 4 groups (NPL per thread, loaded while
-    // staging: the B tile is shared by every row block, so it is usually an L2 hit); rows past n
-    // read row n - 1 (their results are not stored)
+    // staging pieces of 16 bytes: piece w = 4 pl + g of row r is the NPL * 64-byte block of the
+    // step (x6_at).  A: 128 rows x 4 NPL pieces (2 NPL per thread, fetched into registers one
+    // chunk ahead), B: 64 columns x 4 NPL pieces (NPL per thread, loaded while staging: the B
+    // tile is shared by every row block, so it is usually an L2 hit); consecutive threads read
+    // consecutive pieces.  Rows past n read row n - 1 (their results are not stored)
+    constexpr int PW = 4 * NPL;  // pieces per row and step
     u32x4 ra[2 * NPL];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
-            const int q = tid + 256 * u, pl = q >> 9, r = (q >> 2) & 127, g = q & 3;
+            const int q = tid + 256 * u, r = q / PW, w = q % PW;
             const int64_t srow = row0 + r < n ? row0 + r : n - 1;
-            ra[u] = *reinterpret_cast<const u32x4*>(apl + pl * a_plane + srow * kpad + k0 + 8 * g);
+            ra[u] = *reinterpret_cast<const u32x4*>(apl + (srow * ksteps + k0 / X6_KC) * (NPL * X6_KC) + 8 * w);
         }
     };
     auto stash = [&](int k0) {
         u32x4 rb[NPL];
 #pragma unroll
         for (int u = 0; u < NPL; ++u) {
-            const int q = tid + 256 * u, pl = q >> 8, c = (q >> 2) & 63, g = q & 3;
-            rb[u] = *reinterpret_cast<const u32x4*>(bpl + pl * b_plane + (int64_t)(col0 + c) * kpad + k0 + 8 * g);
+            const int q = tid + 256 * u, c = q / PW, w = q % PW;
+            rb[u] = *reinterpret_cast<const u32x4*>(bpl + ((int64_t)(col0 + c) * ksteps + k0 / X6_KC) * (NPL * X6_KC) +
+                                                    8 * w);
         }
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
-            const int q = tid + 256 * u, pl = q >> 9, r = (q >> 2) & 127, g = q & 3;
+            const int q = tid + 256 * u, r = q / PW, w = q % PW, pl = w / 4, g = w % 4;
             *reinterpret_cast<u32x4*>(As + ((pl * X6_TM + r) * 4 + x6_slot(r, g)) * 8) = ra[u];
         }
 #pragma unroll
         for (int u = 0; u < NPL; ++u) {
-            const int q = tid + 256 * u, pl = q >> 8, c = (q >> 2) & 63, g = q & 3;
+            const int q = tid + 256 * u, c = q / PW, w = q % PW, pl = w / 4, g = w % 4;
             *reinterpret_cast<u32x4*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rb[u];
         }
     };
@@ -765,9 +777,9 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
 }
 
 // FP16x3 operand preparation.  Gamma: colscale[c] = max_k max(|Re|, |Im|) of lambda[k] gamma[k][c]
-// (1 for an all-zero column), then planes[p][c][k] of the two fp16 parts of that entry divided
-// by colscale[c].  Carried vectors: one wave per row, rowscale[r] = the row's largest component
-// (1 for a zero row), planes[p][r][k] of the parts of pre[r][k] / rowscale[r].
+// (1 for an all-zero column), then the two fp16 parts of that entry divided by colscale[c] at
+// x6_at(c, k, kpad, 4, p).  Carried vectors: one wave per row, rowscale[r] = the row's largest
+// component (1 for a zero row), the parts of pre[r][k] / rowscale[r] at x6_at(r, k, kpad, 4, p).
 __global__ void colscale_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
                                        int ncols, int ncols_pad, float* __restrict__ colscale) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncols_pad; c += gridDim.x * blockDim.x) {
@@ -785,7 +797,7 @@ __global__ void colscale_fp16x3_kernel(const float* __restrict__ gamma, const do
 __global__ void split_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam,
                                     const float* __restrict__ colscale, int chi_prev, int ncols, int kpad, int ncols_pad,
                                     _Float16* __restrict__ planes) {
-    const int64_t total = (int64_t)ncols_pad * kpad, plane = total;
+    const int64_t total = (int64_t)ncols_pad * kpad;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(i / kpad), k = (int)(i % kpad);
         float2 g = make_float2(0.f, 0.f);
@@ -796,8 +808,8 @@ __global__ void split_fp16x3_kernel(const float* __restrict__ gamma, const doubl
             g.x = (float)(g.x * l) / sc;
             g.y = (float)(g.y * l) / sc;
         }
-        split2_fp16(g.x, planes[i], planes[plane + i]);
-        split2_fp16(g.y, planes[2 * plane + i], planes[3 * plane + i]);
+        split2_fp16(g.x, planes[x6_at(c, k, kpad, 4, 0)], planes[x6_at(c, k, kpad, 4, 1)]);
+        split2_fp16(g.y, planes[x6_at(c, k, kpad, 4, 2)], planes[x6_at(c, k, kpad, 4, 3)]);
     }
 }
 
@@ -805,7 +817,6 @@ __global__ __launch_bounds__(256) void split_rows_fp16x3_kernel(const float* __r
                                                                 int kpad, _Float16* __restrict__ planes,
                                                                 float* __restrict__ rowscale) {
     const int lane = threadIdx.x % 64;
-    const int64_t plane = n * kpad;
     for (int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; r < n; r += (int64_t)gridDim.x * 4) {
         const float2* row = reinterpret_cast<const float2*>(pre) + r * chi_prev;
         float m = 0.f;
@@ -815,9 +826,8 @@ __global__ __launch_bounds__(256) void split_rows_fp16x3_kernel(const float* __r
         if (lane == 0) rowscale[r] = sc;
         for (int k = lane; k < kpad; k += 64) {
             const float2 v = k < chi_prev ? row[k] : make_float2(0.f, 0.f);
-            const int64_t i = r * kpad + k;
-            split2_fp16(v.x / sc, planes[i], planes[plane + i]);
-            split2_fp16(v.y / sc, planes[2 * plane + i], planes[3 * plane + i]);
+            split2_fp16(v.x / sc, planes[x6_at(r, k, kpad, 4, 0)], planes[x6_at(r, k, kpad, 4, 1)]);
+            split2_fp16(v.y / sc, planes[x6_at(r, k, kpad, 4, 2)], planes[x6_at(r, k, kpad, 4, 3)]);
         }
     }
 }

@@ -214,13 +214,14 @@ int bsmps_site_contract_bf16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi,
 /* BF16x6 form of the contraction, FP32-accurate on bf16 MFMA (no reference counterpart):
  * each FP32 value is split exactly into three bf16 parts h + m + l and a product keeps the
  * six terms hh + hm + mh + hl + mm + lh (dropped terms ~2^-24 relative), FP32 accumulation.
- * Both operands are passed split, as six bf16 planes (parts h, m, l of the real part, then of
- * the imaginary part) with k contiguous and padded with zeros to kpad = round_up(chi_prev, 32):
- *   bsmps_split_bf16x6: lambda_prev[k] * gamma[k][c] (gamma complex64 [chi_prev][chi*d]) as
- *     planes [6][round_up(chi*d, 64)][kpad], once per site; bsmps_bf16x6_plane_bytes gives
- *     their size;
- *   bsmps_split_rows_bf16x6: the carried vectors pre (complex64 [n][chi_prev]) as planes
- *     [6][n][kpad], once per step; bsmps_bf16x6_row_plane_bytes gives their size.
+ * Both operands are passed split into six bf16 parts (h, m, l of the real part, then of the
+ * imaginary part), k padded with zeros to kpad = round_up(chi_prev, 32), laid out per row in
+ * 32-k steps: part p of element k of row r at [r][k / 32][p][k % 32] (one step of all parts of
+ * a row is 384 contiguous bytes):
+ *   bsmps_split_bf16x6: lambda_prev[k] * gamma[k][c] (gamma complex64 [chi_prev][chi*d]), rows
+ *     c < round_up(chi*d, 64), once per site; bsmps_bf16x6_plane_bytes gives their size;
+ *   bsmps_split_rows_bf16x6: the carried vectors pre (complex64 [n][chi_prev]), rows r < n,
+ *     once per step; bsmps_bf16x6_row_plane_bytes gives their size.
  * The size functions return -1 for invalid extents.  T is complex64 [n][chi*d] as in
  * bsmps_site_contract_c64. */
 int64_t bsmps_bf16x6_plane_bytes(int chi_prev, int chi, int d);
@@ -236,9 +237,9 @@ int bsmps_site_contract_bf16x6(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi,
  * parts h = fp16(x), l = fp16(x - h) (22 significand bits); a product keeps hh + hl + lh on fp16
  * MFMA (v_mfma_f32_16x16x32_f16) with FP32 accumulation, and the epilogue multiplies the scales
  * back (relative error ~2^-21 per product, three MFMAs per real product instead of six).
- * Planes as in the BF16x6 form with two parts: [4][round_up(chi*d, 64)][kpad] (once per site,
- * with colscale float [round_up(chi*d, 64)]) and [4][n][kpad] (once per step, with rowscale
- * float [n]). */
+ * Layout as in the BF16x6 form with four parts (h, l of the real part, then of the imaginary
+ * part): [round_up(chi*d, 64)][kpad / 32][4][32] (once per site, with colscale float
+ * [round_up(chi*d, 64)]) and [n][kpad / 32][4][32] (once per step, with rowscale float [n]). */
 int64_t bsmps_fp16x3_plane_bytes(int chi_prev, int chi, int d);
 int64_t bsmps_fp16x3_row_plane_bytes(int64_t n, int chi_prev);
 int bsmps_split_fp16x3(bsmps_ctx *ctx, int chi_prev, int chi, int d, const float *gamma,

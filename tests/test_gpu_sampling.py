@@ -206,7 +206,7 @@ def _bf16x6_contract(ctx, pre, lam, G):
     ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G_d), runtime.ptr(lam_d), runtime.ptr(planes))
     ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre_d), runtime.ptr(rows))
     # the three parts of every value add up to it exactly
-    parts = rows.view(6, n, kpad).float()
+    parts = rows.view(n, kpad // 32, 6, 32).permute(2, 0, 1, 3).reshape(6, n, kpad).float()
     re = (parts[0].double() + parts[1].double() + parts[2].double())[:, :chi_p]
     im = (parts[3].double() + parts[4].double() + parts[5].double())[:, :chi_p]
     assert torch.equal(re.float(), pre_d.real) and torch.equal(im.float(), pre_d.imag)
@@ -276,7 +276,7 @@ def _fp16x3_contract(ctx, pre, lam, G):
     # within fp16's 22-bit two-part precision
     want_rs = np.maximum(np.abs(pre.real), np.abs(pre.imag)).max(axis=1)
     assert np.allclose(runtime.to_host(rowscale), want_rs, rtol=0, atol=0)
-    parts = rows.view(4, n, kpad).double()
+    parts = rows.view(n, kpad // 32, 4, 32).permute(2, 0, 1, 3).reshape(4, n, kpad).double()
     re = (parts[0] + parts[1])[:, :chi_p] * rowscale.double()[:, None]
     assert torch.abs(re - pre_d.real.double()).max() <= 2.0 ** -21 * float(want_rs.max())
     T = torch.empty((n, chi, d), dtype=torch.complex64, device=ctx.device)
