@@ -498,7 +498,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from bosonsupremacy_amd import instances, kron, mps, runtime, sampling
+    from bosonsupremacy_amd import batched, instances, kron, mps, runtime, sampling
 
     inst = instances.workload(args.workload)
     n_samples = args.samples or inst.n_samples
@@ -601,6 +601,13 @@ def main():
         s.run(normals, uniforms, timer=ft)
         t_contract = sum(ft.summary().get("site_contract", [0.0]))
         smp, probs = s.run(normals, uniforms, return_probs=True)
+        if prec != "fp64":
+            # along the FP64 sample paths: each uniform moved to the middle of the FP64 draw's
+            # cdf interval, so a draw that flips at a cdf boundary (sampling, not arithmetic)
+            # does not send the two forms down different paths
+            draws_differing = int((smp != ref["samples"]).any(dim=1).sum())
+            smp, probs = s.run(normals, ref["mid"], return_probs=True)
+            off_path = int((smp != ref["samples"]).any(dim=1).sum())
         rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
         e2 = metrics.rb_second_order(smp, probs)
         # complex multiply-adds of all site contractions; MFMA flops executed per one of them:
@@ -615,7 +622,7 @@ def main():
                        "contract_mfma_utilisation": per * cmacs / t_contract / 1e12 / peak}
         log(f"sampling form {prec}: {n_samples / dt:.0f} samples/s")
         if prec == "fp64":
-            ref = {"e2": e2, "samples": smp}
+            ref = {"e2": e2, "samples": smp, "mid": batched.mid_uniforms(smp, probs)}
             # the FP64 MPS against the exact lossy Gaussian state (chi/d truncation + sampling noise)
             mean_x, cov_x = metrics.exact_photon_moments(inst.cov)
             e1 = rb[prec].cpu().numpy()
@@ -631,6 +638,8 @@ def main():
             forms[prec]["second_order_marginal_max_rel_err_vs_fp64"] = float(
                 np.max(np.abs(e2[big] - ref["e2"][big]) / np.abs(ref["e2"][big])))
             forms[prec]["pair_tvd_vs_fp64"] = metrics.pair_tvd(smp, ref["samples"], inst.dd, pairs)
+            forms[prec]["draws_differing_same_uniforms"] = draws_differing
+            forms[prec]["off_fp64_path"] = off_path
             forms[prec]["speedup_vs_fp64"] = forms[prec]["samples_per_s"] / forms["fp64"]["samples_per_s"]
         del probs, smp
 

@@ -61,7 +61,7 @@ class GraphSampler:
         return self.out
 
 
-def _mid_uniforms(samples, probs):
+def mid_uniforms(samples, probs):
     """Uniforms [M, n] in the middle of each draw's cumulative-probability interval (the draw
     is the number of cdf entries below the uniform), so a pass with them repeats ``samples``."""
     idx = samples.T.long().unsqueeze(2)                                  # [M, n, 1]
@@ -107,7 +107,7 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
         ta = now()
         s64, p64 = graphs["fp64"].run(normals, uniforms)
         times["sampling_fp64"] = now() - ta
-        mid = _mid_uniforms(s64, p64)
+        mid = mid_uniforms(s64, p64)
         m64 = p64.mean(dim=1)
         line = []
         for f in forms:

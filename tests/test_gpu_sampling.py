@@ -222,8 +222,9 @@ def test_site_contract_bf16x6_matches_matmul():
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(9)
+    # the last shape has k >= 2048 (128-column tiles) and a last tile past the padded columns
     for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
-                                     (129, 100, 30, 6, 6)):
+                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2)):
         scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
         scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
         pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
@@ -292,8 +293,9 @@ def test_site_contract_fp16x3_matches_matmul():
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(10)
+    # the last shape has k >= 2048 (128-column tiles) and a last tile past the padded columns
     for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
-                                     (129, 100, 30, 6, 6)):
+                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2)):
         scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
         scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
         pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
