@@ -69,10 +69,17 @@ def mid_uniforms(samples, probs):
     return (cdf.gather(2, idx) - 0.5 * probs.gather(2, idx)).squeeze(2).contiguous()
 
 
-def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), verbose=False):
+def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), verbose=False,
+              learned_model=None, learned_precision="fp16x3"):
     """Construct and sample every instance; returns a summary dict (progress lines go to
-    stderr when ``verbose``).  Each form in ``forms`` is timed and checked against FP64."""
+    stderr when ``verbose``).  Each form in ``forms`` is timed and checked against FP64.
+
+    With ``learned_model`` (learned.MLP) every instance is also run in the learned-cutoff form:
+    its spectra, the exactly judged plan (learned.plan_sites), the per-site-shaped MPS and one
+    graph-replayed pass in ``learned_precision``, timed as a whole ("learned" instances/s) and
+    checked along the FP64 sample paths of the plain MPS."""
     from ._lib import BsmpsError
+    from . import kron as kron_mod, learned as learned_mod
 
     def now():
         torch.cuda.synchronize()
@@ -83,6 +90,9 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
     for f in names:
         tot["setup_" + f] = tot["sampling_" + f] = 0.0
     errs = {f: [] for f in forms}
+    if learned_model is not None:
+        tot.update({"construction_learned": 0.0, "setup_learned": 0.0, "sampling_learned": 0.0})
+        errs["learned"] = []
     rows, skipped = [], []
     for i, inst in enumerate(instances):
         t0 = now()
@@ -122,6 +132,25 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
             row.update({f"{f}_marginal_abs_err": err, f"{f}_draws_differing_same_uniforms": flips,
                         f"{f}_off_fp64_path": off_path})
             line.append(f"{f} {times['sampling_' + f]:.3f} s err {err:.1e} flips {flips}")
+        if learned_model is not None:
+            ta = now()
+            ko = kron_mod.kron_all(inst.sq_cov, inst.d, inst.chi)
+            site_d, site_chi, plan = learned_mod.plan_sites(learned_model, inst.sq_cov, inst.cov, ko[0],
+                                                            inst.d, inst.chi)
+            Gl, Ll, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=site_d, site_chi=site_chi)
+            tb = now()
+            gl = GraphSampler(Gl, Ll, sqrtW, inst.dd, n_samples, learned_precision, return_probs=True)
+            tc = now()
+            gl.run(normals, uniforms)
+            td = now()
+            times.update({"construction_learned": tb - ta, "setup_learned": tc - tb, "sampling_learned": td - tc})
+            sl, pl = gl.run(normals, mid)
+            err = float(torch.abs(pl.mean(dim=1) - m64).max())
+            errs["learned"].append(err)
+            row.update({"learned_marginal_abs_err": err, "learned_off_fp64_path": int((sl != s64).any(dim=1).sum()),
+                        "learned_mean_cutoff": plan["mean_cutoff"], "learned_mean_rank": plan["mean_rank"]})
+            line.append(f"learned+{learned_precision} construction {tb - ta:.2f} s err {err:.1e}")
+            del Gl, Ll, gl
         for k, v in times.items():
             tot[k] += v
         row.update({k + "_s": v for k, v in times.items()})
@@ -136,6 +165,12 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
     for f in names:
         out[f"instances_per_s_{f}"] = done / (tot["construction"] + tot["setup_" + f] + tot["sampling_" + f])
     out.update({k + "_s": v for k, v in tot.items()})
+    if learned_model is not None:
+        out[f"instances_per_s_learned_{learned_precision}"] = done / (
+            tot["construction_learned"] + tot["setup_learned"] + tot["sampling_learned"])
+        out["learned_marginal_abs_err_max"] = float(np.max(errs["learned"]))
+        out["learned_marginal_abs_err_median"] = float(np.median(errs["learned"]))
+        out["learned_off_fp64_path_total"] = int(sum(r["learned_off_fp64_path"] for r in rows))
     for f in forms:
         out[f"{f}_marginal_abs_err_max"] = float(np.max(errs[f]))
         out[f"{f}_marginal_abs_err_median"] = float(np.median(errs[f]))

@@ -57,3 +57,18 @@ def test_small_random_batch_runs():
         assert out[f"{f}_marginal_abs_err_max"] < 1e-6
         assert out[f"{f}_off_fp64_path_total"] == 0
         assert out[f"instances_per_s_{f}"] > 0
+
+
+def test_small_random_batch_learned_form():
+    """The learned-cutoff form in the batched driver: per-site-shaped MPS, graph-replayed FP16x3
+    pass, marginals along the plain FP64 sample paths within the 1e-6 bar."""
+    from bosonsupremacy_amd import batched, instances, learned
+    insts = instances.random_batch(2, seed=11, n_samples=500)
+    for inst in insts:
+        inst.chi = min(inst.chi, 128)
+    out = batched.run_batch(insts, n_samples=500, learned_model=learned.load_or_train())
+    print({k: v for k, v in out.items() if k != "per_instance"}, flush=True)
+    assert out["instances"] >= 1
+    assert out["instances_per_s_learned_fp16x3"] > 0
+    assert out["learned_marginal_abs_err_max"] < 1e-6
+    assert out["learned_off_fp64_path_total"] <= out["instances"] * 500 // 100
