@@ -98,7 +98,7 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
     for (int m = 0; m <= BSMPS_MAX_HAFNIAN_SIZE / 2; ++m) c->split_by_m[m] = m >= 6 ? 1 : 0;
     c->hws = nullptr;
     c->hws_bytes = 0;
-    c->split_chunk_bytes = (size_t)4 << 30;
+    c->split_chunk_bytes = (size_t)16 << 30;
     c->contract_gauss[0] = 0;
     c->contract_gauss[1] = 0;
     for (int i = 0; i < bsmps_ctx::kPinSlots; ++i) {

@@ -790,6 +790,7 @@ __global__ __launch_bounds__(kWave) void haf_subtree_kernel(BfsData bd, const in
 // Frontier split, last phase: add the subtree values in the order of the recursion tree
 // (comb: f >= 0 pushes subtree f, -1 adds the top two) and apply the item's epilogue.
 constexpr int kCombineStack = 18;
+constexpr int kCombineBatch = 16;
 template <class S>
 __global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0, int64_t n_h, int64_t stride,
                                                             const double* __restrict__ vals,
@@ -798,15 +799,28 @@ __global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0
     const int lane = threadIdx.x;
     const int64_t h = (int64_t)blockIdx.x * kWave + lane;
     if (h >= n_h) return;
+    // the values of the next kCombineBatch pushes are loaded before any is used, so a lane has
+    // that many loads in flight instead of one (the program, uniform across lanes, is the same)
     int sp = 0;
-    for (int i = 0; i < n_comb; ++i) {
-        const int c = comb[i];
-        if (c >= 0) {
-            stack[sp * kWave + lane] = ld_cd(vals, (int64_t)c * stride + h);
-            ++sp;
-        } else {
-            stack[(sp - 2) * kWave + lane] = cadd(stack[(sp - 2) * kWave + lane], stack[(sp - 1) * kWave + lane]);
-            --sp;
+    for (int i0 = 0; i0 < n_comb; i0 += kCombineBatch) {
+        cd v[kCombineBatch];
+#pragma unroll
+        for (int u = 0; u < kCombineBatch; ++u) {
+            const int c = i0 + u < n_comb ? comb[i0 + u] : -1;
+            v[u] = c >= 0 ? ld_cd(vals, (int64_t)c * stride + h) : czero();
+        }
+#pragma unroll
+        for (int u = 0; u < kCombineBatch; ++u) {
+            if (i0 + u < n_comb) {
+                if (comb[i0 + u] >= 0) {
+                    stack[sp * kWave + lane] = v[u];
+                    ++sp;
+                } else {
+                    stack[(sp - 2) * kWave + lane] =
+                        cadd(stack[(sp - 2) * kWave + lane], stack[(sp - 1) * kWave + lane]);
+                    --sp;
+                }
+            }
         }
     }
     const cd v = stack[lane];

@@ -60,7 +60,7 @@ int bsmps_set_group_lanes(bsmps_ctx *ctx, int n, int group_lanes);
  * evaluated by a one-lane-per-subtree kernel; results are identical to the unsplit path (no
  * reference counterpart). */
 int bsmps_set_split(bsmps_ctx *ctx, int n, int enable);
-/* Device scratch the split path may hold (default 4 GiB, at least 1 MiB): batches of
+/* Device scratch the split path may hold (default 16 GiB, at least 1 MiB): batches of
  * hafnians are processed in chunks whose frontier data fits it, with at least 64 hafnians
  * per chunk (one wavefront), which alone needs 2.2 GB at size 30, 4.8 GB at 32 and 10.3 GB
  * at 34. */

@@ -19,7 +19,7 @@ def ctx():
     c.set_tuning(4, 64)
     for n in range(12, 31, 2):
         c.call("bsmps_set_split", n, 1)
-    c.call("bsmps_set_split_budget", 4 << 30)
+    c.call("bsmps_set_split_budget", 16 << 30)
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -111,7 +111,7 @@ def test_split_matches_unsplit_and_oracle(ctx, n):
     ctx.call("bsmps_set_split_budget", 1 << 20)
     print(f"n={n} split, 1 MiB budget", flush=True)
     chunked, _, _ = mps.A_elem(S, tg, den)
-    ctx.call("bsmps_set_split_budget", 4 << 30)
+    ctx.call("bsmps_set_split_budget", 16 << 30)
     ref = outs.get((0, 64), outs[(1, 64)])
     for key, vals in outs.items():
         assert same_bits(vals, ref), key
