@@ -253,7 +253,7 @@ def partial_run(args):
     t_kron = time.perf_counter() - t0
     log(f"spectra of {inst.M - 1} cuts: {t_kron:.2f} s")
     work = site_hafnian_work_by_site(ko[0], ko[1], inst.d)
-    # the learned-cutoff plan of the whole chain: what the exact judge lets it drop here
+    # the learned-cutoff plan of the whole chain: what the exact discarded-weight check lets it drop
     from bosonsupremacy_amd import learned
     site_d, site_chi, plan = learned.plan_sites(learned.load_or_train(), inst.sq_cov, inst.cov, ko[0],
                                                 inst.d, inst.chi)
