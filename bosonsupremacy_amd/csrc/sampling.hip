@@ -126,14 +126,18 @@ struct MfmaTraits<float> {
 };
 
 template <typename E>
-__global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
+__global__ __launch_bounds__(256, 2) void site_gemm_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
                                                         const E* __restrict__ pre, const double* __restrict__ lam,
                                                         const E* __restrict__ gamma, E* __restrict__ T) {
     typedef MfmaTraits<E> Tr;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::pair_t E2;
-    __shared__ E As[TK][TM + 2];  // As[k][row]
-    __shared__ E Bs[TK][TN + 2];  // Bs[k][col]
+    // real k per LDS stage: FP64 takes 32, so a stage's MFMA work (~128 per wave) covers the
+    // next stage's prefetch (0.685 -> 0.726 of the FP64 MFMA peak at n = 10^5, chi = 512, d = 6);
+    // FP32 MFMAs are twice as fast and its registers prefer 16 (32 measured 0.72 -> 0.67)
+    constexpr int TKB = sizeof(E) == 8 ? 32 : 16;
+    __shared__ E As[TKB][TM + 2];  // As[k][row]
+    __shared__ E Bs[TKB][TN + 2];  // Bs[k][col]
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int64_t row0 = (int64_t)blockIdx.y * TM;
@@ -148,13 +152,14 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    // per thread: 4 complex of the A tile (128 rows x 8 b) and 2 of the B tile (8 b x 64 c)
-    E2 ra[4], rb[2];
+    // per thread: QA complex of the A tile (128 rows x TKB/2 b) and QB of the B tile (TKB/2 b x 64 c)
+    constexpr int QA = TM * (TKB / 2) / 256, QB = (TKB / 2) * (TN / 2) / 256;
+    E2 ra[QA], rb[QB];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
-            const int r = e / (TK / 2), bb = e % (TK / 2);
+            const int r = e / (TKB / 2), bb = e % (TKB / 2);
             const int64_t srow = row0 + r;
             const int b = k0 / 2 + bb;
             E2 v;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
             ra[q] = v;
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
             const int bb = e / (TN / 2), cc = e % (TN / 2);
             const int b = k0 / 2 + bb;
@@ -183,14 +188,14 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
-            const int r = e / (TK / 2), bb = e % (TK / 2);
+            const int r = e / (TKB / 2), bb = e % (TKB / 2);
             As[2 * bb][r] = ra[q].x;
             As[2 * bb + 1][r] = ra[q].y;
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
             const int bb = e / (TN / 2), cc = e % (TN / 2);
             Bs[2 * bb][2 * cc] = rb[q].x;
@@ -201,12 +206,12 @@ __global__ __launch_bounds__(256) void site_gemm_kernel(int64_t n, int chi_prev,
     };
 
     fetch(0);
-    for (int k0 = 0; k0 < Kreal; k0 += TK) {
+    for (int k0 = 0; k0 < Kreal; k0 += TKB) {
         stash();
         __syncthreads();
-        if (k0 + TK < Kreal) fetch(k0 + TK);
+        if (k0 + TKB < Kreal) fetch(k0 + TKB);
 #pragma unroll
-        for (int kk = 0; kk < TK; kk += 4) {
+        for (int kk = 0; kk < TKB; kk += 4) {
             E a[4], b[4];
             // operand layout: lane holds A[lane%16][lane/16] and B[lane/16][lane%16]
 #pragma unroll
