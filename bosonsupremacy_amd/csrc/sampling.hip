@@ -567,6 +567,7 @@ struct Bf16x6Fmt {  // three bf16 parts (exact), products of order >= 2^-16: six
     static constexpr int NP = 3;
     static constexpr bool SCALED = false;
     static constexpr int MIN_WAVES = 2;  // waves per SIMD the registers are budgeted for
+    static constexpr bool PREFETCH_B = false;  // B fetched one step ahead spills at 6 parts
     __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
     __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
         acc = mfma(a[2], b[0], acc);
@@ -584,6 +585,7 @@ struct Fp16x3Fmt {  // two fp16 parts (22 bits) of values scaled to max 1: three
     static constexpr int NP = 2;
     static constexpr bool SCALED = true;
     static constexpr int MIN_WAVES = 2;  // at 3 the kernel spills
+    static constexpr bool PREFETCH_B = true;  // B fetched one step ahead, like A
     __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
     __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
         acc = mfma(a[1], b[0], acc);
@@ -697,6 +699,15 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
     // consecutive pieces.  Rows past n read row n - 1 (their results are not stored)
     constexpr int PW = 4 * NPL;  // pieces per row and step
     u32x4 ra[2 * NPL];
+    u32x4 rbp[F::PREFETCH_B ? NPL : 1];
+    auto load_b = [&](int k0, u32x4* dst) {
+#pragma unroll
+        for (int u = 0; u < NPL; ++u) {
+            const int q = tid + 256 * u, c = q / PW, w = q % PW;
+            dst[u] = *reinterpret_cast<const u32x4*>(bpl + ((int64_t)(col0 + c) * ksteps + k0 / X6_KC) * (NPL * X6_KC) +
+                                                     8 * w);
+        }
+    };
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
@@ -704,14 +715,15 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
             const int64_t srow = row0 + r < n ? row0 + r : n - 1;
             ra[u] = *reinterpret_cast<const u32x4*>(apl + (srow * ksteps + k0 / X6_KC) * (NPL * X6_KC) + 8 * w);
         }
+        if constexpr (F::PREFETCH_B) load_b(k0, rbp);
     };
     auto stash = [&](int k0) {
         u32x4 rb[NPL];
+        if constexpr (F::PREFETCH_B) {
 #pragma unroll
-        for (int u = 0; u < NPL; ++u) {
-            const int q = tid + 256 * u, c = q / PW, w = q % PW;
-            rb[u] = *reinterpret_cast<const u32x4*>(bpl + ((int64_t)(col0 + c) * ksteps + k0 / X6_KC) * (NPL * X6_KC) +
-                                                    8 * w);
+            for (int u = 0; u < NPL; ++u) rb[u] = rbp[u];
+        } else {
+            load_b(k0, rb);
         }
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
