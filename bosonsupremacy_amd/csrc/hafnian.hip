@@ -799,8 +799,8 @@ __global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0
     const int lane = threadIdx.x;
     const int64_t h = (int64_t)blockIdx.x * kWave + lane;
     if (h >= n_h) return;
-    // the values of the next kCombineBatch pushes are loaded before any is used, so a lane has
-    // that many loads in flight instead of one (the program, uniform across lanes, is the same)
+    // the values pushed by the next kCombineBatch program steps are loaded before any is used,
+    // so a lane has that many loads in flight instead of one (the program is uniform across lanes)
     int sp = 0;
     for (int i0 = 0; i0 < n_comb; i0 += kCombineBatch) {
         cd v[kCombineBatch];
