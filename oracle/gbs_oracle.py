@@ -38,9 +38,15 @@ Pinning (what checks each function against something other than itself):
 * williamson / blochmessiah / get_U2_sq_U1 / get_Sigma: symplectic invariants
   (S Omega S^T = Omega, V = S Db S^T, unitarity) and golden vectors from the reference's
   functions (the gauge -- mode order and signs -- is only pinned by the latter);
-* site tensors: right-canonical identity and the product-state closed form;
-* sampling: closed-form first-order marginals (tr V_ii - 2)/4 and the spectral form of the
-  displacement exponential.
+* site tensors (build_site, build_mps): tests/test_oracle_pins.py -- the contracted MPS of a
+  small weakly squeezed instance equals, amplitude by amplitude, the Taylor-coefficient form
+  of the pure Gaussian state exp(a^dag B a^dag / 2)|0> / det(Q)^(1/4) (within the truncated
+  weight, 2.6e-6); every site tensor satisfies the right- and left-canonical identities
+  (sums <= 1, weighted deficits ~1e-6); a dropped sqrt(j!) or a transposed Gamma fails both;
+* sampling: the same file -- every conditional distribution drawn from equals that of the
+  displaced pure state D(alpha)|psi> given the sample's alpha and earlier draws (to 1e-6; a
+  transposed displacement matrix fails); plus closed-form first-order marginals
+  (tr V_ii - 2)/4 and the spectral form of the displacement exponential.
 """
 import numpy as np
 from scipy.special import factorial
