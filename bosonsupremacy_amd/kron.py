@@ -92,6 +92,8 @@ def main(argv=None):
     parser.add_argument('--d', type=int, required=True, help='local photon cutoff before displacement')
     parser.add_argument('--chi', type=int, required=True, help='bond dimension')
     parser.add_argument('--dir', type=str, required=True, help='root directory (with trailing slash)')
+    parser.add_argument('--reference-dtypes', action='store_true',
+                        help='save res as float16 and num as int8, as kron_cpu.py:127 does (default: res float64)')
     args = parser.parse_args(argv)
     path = args.dir + f"d_{args.d}_chi_{args.chi}/"
     sq_cov = np.load(args.dir + "sq_cov.npy")
@@ -102,8 +104,8 @@ def main(argv=None):
     res, num, S = kron_all(sq_cov, args.d, args.chi)
     for k in range(M - 1):
         print(k, np.sum(res[k]))
-        np.save(path + f'res_{k}.npy', res[k])
-        np.save(path + f'num_{k}.npy', num[k])
+        np.save(path + f'res_{k}.npy', res[k].astype(np.float16) if args.reference_dtypes else res[k])
+        np.save(path + f'num_{k}.npy', num[k].astype(np.int8))
         np.save(path + f'S_{k}.npy', S[k])
 
 

@@ -17,6 +17,7 @@ CLI (same flags and files as the reference)::
 import argparse
 import contextlib
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -308,18 +309,27 @@ class HostStream:
 
 
 def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=None, site_chi=None,
-              to_host=False, sites=None, right_runs=None, on_run=None):
+              to_host=False, sites=None, right_runs=None, on_run=None, pruned=False, consume=None):
     """All site tensors: (list of device Gamma_k, list of host Lambda_k), plus the kron
     outputs used.  ``kron`` = (res, num, S) lists as kron.kron_all returns them; when it is
     None the spectra are computed too.
 
+    ``pruned`` builds the plain method without the work a zero Lambda multiplies away: each
+    Gamma_k is [r_{k-1}][r_k][d] over the kept labels of both cuts (Schmidt weight > 1e-10),
+    where the reference (MPS_cpu.py:461-520) fills [chi][chi][d] and computes the rows of every
+    label of cut k-1, kept or not.  Its entries are the same values as the plain tensor's live
+    block.
     ``site_d`` (per-site photon cutoffs <= d) and ``site_chi`` (per-cut ranks) select the
-    learned-cutoff form (bosonsupremacy_amd.learned.plan_sites); both None is the plain method.
+    learned-cutoff form (bosonsupremacy_amd.learned.plan_sites); neither, and not ``pruned``,
+    is the plain method.
     ``to_host`` streams every Gamma_k to pinned host memory as soon as it is computed (the
     returned Gammas are then pinned host tensors and the device copies are released);
     ``sites`` builds only the listed sites (None entries elsewhere).
     ``right_runs`` = {site: number of runs} computes those sites in runs of right labels,
     calling ``on_run(site, i, n_runs)`` after each (site_tensor).
+    ``consume(k, Gamma_k, Lambda_k)`` is called as soon as each site is built; the site tensor
+    is then not kept (its list entry is None), so only one is held at a time -- the site-major
+    construct-and-sample pipeline (chain.Chain).
 
     Pipelined: the host FP64 work of every site (Bloch-Messiah, Sigma) is queued on the
     host worker pool as soon as the cut symplectics are known -- before the Schmidt spectra
@@ -338,8 +348,8 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
         site_f = hostpool.submit_sites(sq_cov, kron[2], S_full_f.result())
     kres, knum, kS = kron
     Gammas, Lambdas = [], []
-    learned = site_d is not None or site_chi is not None
-    if learned:
+    shaped = site_d is not None or site_chi is not None or pruned
+    if shaped:
         site_d = list(site_d) if site_d is not None else [d] * M
         if site_chi is None:
             site_chi = [int((np.asarray(r) > ERR_TOL).sum()) for r in kres]
@@ -347,8 +357,8 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
     wanted = set(range(M)) if sites is None else set(sites)
     # the label tables of every site are built by the host workers while the GPU runs the
     # previous sites (a site whose tables are not ready yet builds them here instead)
-    tab_f = hostpool.submit_tables(M, d, chi, kres, knum, ERR_TOL, site_d if learned else None,
-                                   site_chi if learned else None) if sites is None else None
+    tab_f = hostpool.submit_tables(M, d, chi, kres, knum, ERR_TOL, site_d if shaped else None,
+                                   site_chi if shaped else None) if sites is None else None
     for k in range(M):
         if k not in wanted:
             Gammas.append(None)
@@ -359,7 +369,7 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
         if right_runs and right_runs.get(k, 1) > 1:
             runs = {"right_runs": right_runs[k],
                     "on_run": (lambda i, n, k=k: on_run(k, i, n)) if on_run is not None else None}
-        if learned:
+        if shaped:
             ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None)
             G, L = site_tensor(k, M, sq_cov, kres, knum, kS, site_d[k], chi, timer=timer,
                                sigma_z=site_f[k].result(), ranks=ranks, tables=tables, **runs)
@@ -368,6 +378,9 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
                                tables=tables, **runs)
         if streamer is not None:
             G = streamer.push(G)
+        if consume is not None:
+            consume(k, G, L)
+            G = None
         Gammas.append(G)
         Lambdas.append(L)
     if streamer is not None:
@@ -375,11 +388,35 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
     return Gammas, Lambdas, kron
 
 
+def _save_atomic(path, arr):
+    """np.save through a temporary name and a rename, so a run killed mid-write never leaves a
+    truncated file under the final name (resume trusts every file it finds)."""
+    tmp = path + ".partial.npy"
+    np.save(tmp, arr)
+    os.replace(tmp, path)
+
+
+def first_missing_site(path, M):
+    """The first site k whose Gamma_k.npy (or, for k < M-1, Lambda_k.npy) is not in ``path``:
+    where a resumed run starts (MPS_cpu.py:523-525 saves both after each site)."""
+    for k in range(M):
+        if not os.path.exists(path + f"Gamma_{k}.npy"):
+            return k
+        if k < M - 1 and not os.path.exists(path + f"Lambda_{k}.npy"):
+            return k
+    return M
+
+
 def main(argv=None):
     parser = argparse.ArgumentParser(description="MPS site tensors (drop-in for MPS_cpu.py)")
     parser.add_argument('--d', type=int, required=True)
     parser.add_argument('--chi', type=int, required=True)
     parser.add_argument('--dir', type=str, required=True)
+    parser.add_argument('--restart', action='store_true',
+                        help='recompute every site (default: resume at the first site without its files)')
+    parser.add_argument('--reference-dtypes', action='store_true',
+                        help='save Gamma as complex64 and Lambda as float32, as MPS_cpu.py:397-398 does '
+                             '(default: complex128 / float64)')
     args = parser.parse_args(argv)
     d, chi = args.d, args.chi
     path = args.dir + f"d_{d}_chi_{chi}/"
@@ -389,15 +426,19 @@ def main(argv=None):
     kres = [np.load(path + f'res_{k}.npy').astype(np.float64) for k in range(M - 1)]
     knum = [np.load(path + f'num_{k}.npy') for k in range(M - 1)]
     kS = [np.load(path + f'S_{k}.npy') for k in range(M - 1)]
-    site_f = hostpool.submit_sites(sq_cov, kS, hostpool.williamson_S(sq_cov).result())
-    for k in range(M):
+    start = 0 if args.restart else first_missing_site(path, M)
+    if start:
+        print(f'resuming at site {start} (sites 0..{start - 1} found in {path})')
+    site_f = hostpool.submit_sites(sq_cov, kS, hostpool.williamson_S(sq_cov).result()) if start < M else []
+    gdt, ldt = (np.complex64, np.float32) if args.reference_dtypes else (np.complex128, np.float64)
+    for k in range(start, M):
         t0 = time.time()
         G, L = site_tensor(k, M, sq_cov, kres, knum, kS, d, chi, sigma_z=site_f[k].result())
-        Gh = runtime.to_host(G)
+        Gh = runtime.to_host(G).astype(gdt, copy=False)
         print('mode: ', k, 'Total {:.3f}'.format(time.time() - t0))
-        np.save(path + f"Gamma_{k}.npy", Gh)
-        if k < M - 1:
-            np.save(path + f"Lambda_{k}.npy", L)
+        if k < M - 1:  # Lambda first: a Gamma file marks a finished site
+            _save_atomic(path + f"Lambda_{k}.npy", np.asarray(L).astype(ldt, copy=False))
+        _save_atomic(path + f"Gamma_{k}.npy", Gh)
 
 
 if __name__ == "__main__":

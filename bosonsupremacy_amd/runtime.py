@@ -97,12 +97,22 @@ def context(device=0):
 
 
 def ptr(t):
-    """Device pointer of a contiguous tensor (or None -> NULL)."""
+    """Device pointer of a contiguous device tensor (or None -> NULL).  A host tensor -- pinned
+    or not -- is refused: the kernels would read host memory over PCIe, or fault."""
     if t is None:
         return None
+    if not t.is_cuda:
+        raise ValueError("a device tensor is needed here; copy host tensors to the GPU first")
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def host(a):
+    """NumPy view/copy of an array or tensor (device tensors are copied to the host)."""
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
 
 
 def to_device(a, dtype=None, device=None):

@@ -90,14 +90,176 @@ def random_inputs(M, n, seed, offset=0):
     return normals, uniforms
 
 
-class Sampler:
-    """Samples photon numbers from an MPS held as device tensors.
+def live_ranks(gammas, lambdas):
+    """Schmidt ranks the sampler contracts: r_i = 1 + the index of the last non-zero entry of
+    Lambda_i (i < M-1), capped by Gamma_i's column count, and r_{M-1} = 1 (the last site's one
+    column, sampling_cpu.py:88-89).  Rows and columns past these ranks are multiplied by a zero
+    Lambda (sampling_cpu.py:75, 90), so leaving them out changes no probability."""
+    M = len(gammas)
+    ranks = []
+    for i in range(M - 1):
+        lam = lambdas[i].detach().cpu().numpy() if isinstance(lambdas[i], torch.Tensor) else np.asarray(lambdas[i])
+        nz = np.flatnonzero(lam)
+        ranks.append(max(1, min(int(nz[-1]) + 1 if len(nz) else 1, int(gammas[i].shape[1]))))
+    ranks.append(1)
+    return ranks
 
-    gammas : list of M device tensors [chi_{i-1}, chi_i, d_i] complex128 (the plain method:
-             all [chi, chi, d]; site 0 uses row 0 only, so its first extent may be 1)
+
+def _as_tensor(g):
+    return g if isinstance(g, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(g))
+
+
+class _SiteForm:
+    """Device data of one site in one precision: the live block Gamma_i[:rows, :cols, :d] and, for
+    the split forms, its pre-split operand planes (built once per site, reused by every batch)."""
+    __slots__ = ("rows", "cols", "d", "gamma", "split")
+
+
+def _prepare_site(ctx, precision, i, g, rows, cols, lam_prev):
+    """_SiteForm of Gamma_i (device or host tensor, or NumPy array) restricted to its live block;
+    ``lam_prev`` is Lambda_{i-1} on the device (folded into the split forms' planes)."""
+    dev = ctx.device
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    f = _SiteForm()
+    if not isinstance(g, torch.Tensor):   # NumPy (possibly memory-mapped): read the live block only
+        g = np.ascontiguousarray(g[:rows, :cols])
+    g = _as_tensor(g)
+    f.rows, f.cols, f.d = rows, cols, int(g.shape[2])
+    f.gamma = g[:rows, :cols].to(device=dev, dtype=ctype).contiguous()
+    f.split = None
+    if i == 0:
+        return f
+    if precision == "bf16x3":
+        # hi / lo bf16 planes of Gamma_i
+        cnt = 2 * f.gamma.numel()
+        hi = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+        lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+        ctx.call("bsmps_split_bf16", runtime.ptr(f.gamma), cnt, runtime.ptr(hi), runtime.ptr(lo))
+        f.split = (hi, lo)
+    elif precision == "bf16x6":
+        # six bf16 planes of Lambda_{i-1} Gamma_i (FP32-exact split)
+        nbytes = ctx.lib.bsmps_bf16x6_plane_bytes(rows, cols, f.d)
+        planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
+        ctx.call("bsmps_split_bf16x6", rows, cols, f.d, runtime.ptr(f.gamma), runtime.ptr(lam_prev),
+                 runtime.ptr(planes))
+        f.split = planes
+    elif precision == "fp16x3":
+        # two fp16 planes per part of Lambda_{i-1} Gamma_i, columns scaled to a largest component of
+        # 1 (the scales are multiplied back by the contraction)
+        nbytes = ctx.lib.bsmps_fp16x3_plane_bytes(rows, cols, f.d)
+        planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
+        colscale = torch.empty((-(-(cols * f.d) // 64) * 64,), dtype=torch.float32, device=dev)
+        ctx.call("bsmps_split_fp16x3", rows, cols, f.d, runtime.ptr(f.gamma), runtime.ptr(lam_prev),
+                 runtime.ptr(planes), runtime.ptr(colscale))
+        f.split = (planes, colscale)
+    return f
+
+
+def _offset(t, elems):
+    """Device pointer ``elems`` elements into device tensor t."""
+    runtime.ptr(t)  # device and contiguity checks
+    return ctypes.c_void_p(t.data_ptr() + elems * t.element_size())
+
+
+class _Walk:
+    """Device state of one sampling pass over n samples, advanced one site at a time (all
+    samples through site i before site i+1, sampling_cpu.py:54-113): the displacements, the
+    carried vectors, the draws and (optionally) the conditional distributions.
+
+    Samples go through a site in chunks of ``chunk``; with more than one chunk the carried
+    vectors are double-buffered (a chunk's new vectors must not overwrite vectors of chunks
+    that have not been contracted yet)."""
+
+    def __init__(self, ctx, precision, M, dd, sqrtW, W, x, normals, chi_max, cd_max, d_max, chunk=None,
+                 return_probs=False):
+        self.ctx, self.precision, self.M, self.dd = ctx, precision, M, dd
+        self.W, self.x = W, x
+        dev = ctx.device
+        n = normals.shape[1]
+        self.n = n
+        self.chunk = n if chunk is None else max(1, min(int(chunk), n))
+        ctype = torch.complex128 if precision == "fp64" else torch.complex64
+        normals = normals.contiguous()
+        self.alpha = torch.empty((n, M), dtype=torch.complex128, device=dev)
+        ctx.call("bsmps_mu_to_alpha", runtime.ptr(sqrtW), M, runtime.ptr(normals), n, runtime.ptr(self.alpha))
+        self.samples = torch.empty((n, M), dtype=torch.int32, device=dev)
+        n_buf = 1 if self.chunk >= n else 2
+        self.pre = [torch.empty((n * chi_max,), dtype=ctype, device=dev) for _ in range(n_buf)]
+        self.cur = 0                         # the buffer holding the carried vectors
+        self.T = torch.empty((self.chunk * cd_max,), dtype=ctype, device=dev)
+        self.probs = torch.empty((M, n, dd), dtype=torch.float64, device=dev) if return_probs else None
+        self.dwork = torch.empty((2 * self.chunk * dd * d_max,), dtype=torch.float64, device=dev)  # D_s
+        self.pre_planes = self.rowscale = None
+        if precision == "bf16x6":
+            nbytes = ctx.lib.bsmps_bf16x6_row_plane_bytes(self.chunk, chi_max)
+            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
+        if precision == "fp16x3":
+            nbytes = ctx.lib.bsmps_fp16x3_row_plane_bytes(self.chunk, chi_max)
+            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
+            self.rowscale = torch.empty((self.chunk,), dtype=torch.float32, device=dev)
+
+    def site(self, i, f, lam_prev, lam, uniforms_row, timer):
+        """Advance every sample through site i (form ``f``); ``lam_prev`` / ``lam`` are
+        Lambda_{i-1} / Lambda_i on the device, ``uniforms_row`` the [n] uniforms of the site."""
+        ctx, M, dd, prec = self.ctx, self.M, self.dd, self.precision
+        last = i == M - 1
+        draw = "bsmps_sample_site" if prec == "fp64" else "bsmps_sample_site_c64"
+        src_buf = self.pre[self.cur]
+        dst_buf = self.pre[1 - self.cur] if len(self.pre) == 2 else src_buf
+        for c0 in range(0, self.n, self.chunk):
+            nc = min(self.n, c0 + self.chunk) - c0
+            if i == 0:
+                src, stride = f.gamma, 0   # Gamma_0[0, a, j] is the site-0 tensor of every sample
+            else:
+                pre_in = _offset(src_buf, c0 * f.rows)
+                with timer("site_contract"):
+                    self._contract(nc, f, pre_in, lam_prev)
+                src, stride = self.T, f.cols * f.d
+            with timer("sample_site"):
+                ctx.call(draw, nc, f.cols, f.d, dd, runtime.ptr(src), stride, _offset(self.alpha, c0 * M + i), M,
+                         runtime.ptr(self.W), runtime.ptr(self.x), runtime.ptr(self.dwork),
+                         None if last else runtime.ptr(lam), _offset(uniforms_row, c0),
+                         _offset(self.samples, c0 * M + i), M, None if last else _offset(dst_buf, c0 * f.cols),
+                         _offset(self.probs[i], c0 * dd) if self.probs is not None else None)
+        if len(self.pre) == 2:
+            self.cur = 1 - self.cur
+
+    def _contract(self, nc, f, pre_in, lam_prev):
+        ctx, prec = self.ctx, self.precision
+        rows, cols, d = f.rows, f.cols, f.d
+        T = runtime.ptr(self.T)
+        if prec == "fp64":
+            ctx.call("bsmps_site_contract", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(f.gamma), T)
+        elif prec == "fp32":
+            ctx.call("bsmps_site_contract_c64", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev),
+                     runtime.ptr(f.gamma), T)
+        elif prec == "fp16x3":
+            planes, colscale = f.split
+            ctx.call("bsmps_split_rows_fp16x3", nc, rows, pre_in, runtime.ptr(self.pre_planes),
+                     runtime.ptr(self.rowscale))
+            ctx.call("bsmps_site_contract_fp16x3", nc, rows, cols, d, runtime.ptr(self.pre_planes),
+                     runtime.ptr(self.rowscale), runtime.ptr(planes), runtime.ptr(colscale), T)
+        elif prec == "bf16x6":
+            ctx.call("bsmps_split_rows_bf16x6", nc, rows, pre_in, runtime.ptr(self.pre_planes))
+            ctx.call("bsmps_site_contract_bf16x6", nc, rows, cols, d, runtime.ptr(self.pre_planes),
+                     runtime.ptr(f.split), T)
+        else:
+            hi, lo = f.split
+            ctx.call("bsmps_site_contract_bf16x3", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev),
+                     runtime.ptr(hi), runtime.ptr(lo), T)
+
+
+class Sampler:
+    """Samples photon numbers from an MPS whose site tensors are all held on the device.
+
+    gammas : list of M tensors [chi_{i-1}, chi_i, d_i] complex128 (device or host tensors, or
+             NumPy arrays; the plain method: all [chi, chi, d], site 0 uses row 0 only)
     lambdas: list of M float64 arrays [chi_i] (the last one unused)
     sqrtW  : [2M, 2M] Cholesky factor of the classical noise covariance
-    """
+
+    Only the live block of each site tensor is kept and contracted: rows and columns up to
+    the Schmidt ranks of ``live_ranks`` (zero-padded Lambda entries multiply the rest away) and
+    the last site's single column."""
 
     PRECISIONS = ("fp64", "fp32", "bf16x3", "bf16x6", "fp16x3")
 
@@ -105,18 +267,19 @@ class Sampler:
         self.ctx = ctx or runtime.context()
         dev = self.ctx.device
         self.M = len(gammas)
-        self.chis = [int(g.shape[1]) for g in gammas]
+        if len(lambdas) < self.M - 1:
+            raise ValueError("one Lambda per cut is needed")
+        self.chis = live_ranks(gammas, lambdas)
+        self.rows = [1] + self.chis[:-1]
         self.ds = [int(g.shape[2]) for g in gammas]
         self.chi, self.d = max(self.chis), max(self.ds)
         self.dd = dd
         if dd < self.d:
             raise ValueError("dd must be >= d")
-        for i in range(1, self.M):
-            if gammas[i].shape[0] < self.chis[i - 1]:
-                raise ValueError(f"Gamma_{i} has {gammas[i].shape[0]} rows, fewer than the {self.chis[i - 1]} "
-                                 f"Schmidt vectors of the previous cut")
-            if len(lambdas[i - 1]) < self.chis[i - 1]:
-                raise ValueError(f"Lambda_{i - 1} has fewer than {self.chis[i - 1]} entries")
+        for i in range(self.M):
+            if gammas[i].shape[0] < self.rows[i] or gammas[i].shape[1] < self.chis[i]:
+                raise ValueError(f"Gamma_{i} of shape {tuple(gammas[i].shape)} is smaller than its live block "
+                                 f"[{self.rows[i]}, {self.chis[i]}]")
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         self.precision = precision
@@ -124,52 +287,15 @@ class Sampler:
         # context bound to a side stream sees its inputs ready
         self.ctx.stream.wait_stream(torch.cuda.current_stream(dev))
         with torch.cuda.stream(self.ctx.stream):
-            self._prepare(gammas, lambdas, sqrtW, dd, dev)
+            self.lambdas = [runtime.to_device(np.asarray(runtime.host(l) if l is not None else np.zeros(1),
+                                                         dtype=np.float64)) for l in lambdas]
+            self.forms = [_prepare_site(self.ctx, precision, i, gammas[i], self.rows[i], self.chis[i],
+                                        self.lambdas[i - 1] if i else None) for i in range(self.M)]
+            self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
+            W, x = displacement_basis(dd)
+            self.W = runtime.to_device(W)
+            self.x = runtime.to_device(x)
         self.dev = dev
-
-    def _prepare(self, gammas, lambdas, sqrtW, dd, dev):
-        """Device copies of the site tensors in the form's precision (and their bf16 splits)."""
-        precision = self.precision
-        self.lambdas = [runtime.to_device(np.asarray(l, dtype=np.float64)) for l in lambdas]
-        ctype = torch.complex128 if precision == "fp64" else torch.complex64
-        self.gammas = [g.to(ctype).contiguous() for g in gammas]
-        self.splits = None
-        if precision == "bf16x3":
-            # hi / lo bf16 planes of every Gamma, split once and reused by every batch
-            self.splits = []
-            for g in self.gammas:
-                cnt = 2 * g.numel()
-                hi = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
-                lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
-                self.ctx.call("bsmps_split_bf16", runtime.ptr(g), cnt, runtime.ptr(hi), runtime.ptr(lo))
-                self.splits.append((hi, lo))
-        if precision == "bf16x6":
-            # six bf16 planes of every contracted Lambda_{i-1} Gamma_i (FP32-exact split), reused
-            # by every batch
-            self.splits = [None]
-            for i in range(1, self.M):
-                chi_p, chi, d = self.chis[i - 1], self.chis[i], self.ds[i]
-                nbytes = self.ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d)
-                planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
-                self.ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(self.gammas[i]),
-                              runtime.ptr(self.lambdas[i - 1]), runtime.ptr(planes))
-                self.splits.append(planes)
-        if precision == "fp16x3":
-            # two fp16 planes per part of every contracted Lambda_{i-1} Gamma_i, columns scaled to a
-            # largest component of 1 (the scales are multiplied back by the contraction)
-            self.splits = [None]
-            for i in range(1, self.M):
-                chi_p, chi, d = self.chis[i - 1], self.chis[i], self.ds[i]
-                nbytes = self.ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d)
-                planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
-                colscale = torch.empty((-(-(chi * d) // 64) * 64,), dtype=torch.float32, device=dev)
-                self.ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(self.gammas[i]),
-                              runtime.ptr(self.lambdas[i - 1]), runtime.ptr(planes), runtime.ptr(colscale))
-                self.splits.append((planes, colscale))
-        self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
-        W, x = displacement_basis(dd)
-        self.W = runtime.to_device(W)
-        self.x = runtime.to_device(x)
 
     def run(self, normals, uniforms, return_probs=False, timer=None):
         """normals [2M, n], uniforms [M, n] (device float64) -> samples [n, M] int32 (device),
@@ -177,65 +303,117 @@ class Sampler:
         context-manager factory ``timer(name)`` wrapped around each kernel launch."""
         import contextlib
         timer = timer or (lambda name: contextlib.nullcontext())
-        ctx, M, dd = self.ctx, self.M, self.dd
-        n = normals.shape[1]
-        normals = normals.contiguous()
         uniforms = uniforms.contiguous()
-        alpha = torch.empty((n, M), dtype=torch.complex128, device=self.dev)
-        ctx.call("bsmps_mu_to_alpha", runtime.ptr(self.sqrtW), M, runtime.ptr(normals), n, runtime.ptr(alpha))
-        samples = torch.empty((n, M), dtype=torch.int32, device=self.dev)
-        ctype = torch.complex128 if self.precision == "fp64" else torch.complex64
-        draw = "bsmps_sample_site" if self.precision == "fp64" else "bsmps_sample_site_c64"
-        pre = torch.empty((n * self.chi,), dtype=ctype, device=self.dev)
-        T = torch.empty((n * max(c * dk for c, dk in zip(self.chis, self.ds)),), dtype=ctype, device=self.dev)
-        probs = torch.empty((M, n, dd), dtype=torch.float64, device=self.dev) if return_probs else None
-        dwork = torch.empty((2 * n * dd * self.d,), dtype=torch.float64, device=self.dev)  # D_s per sample
-        pre_planes = rowscale = None
-        if self.precision == "bf16x6":
-            nbytes = max(self.ctx.lib.bsmps_bf16x6_row_plane_bytes(n, c) for c in self.chis)
-            pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=self.dev)
-        if self.precision == "fp16x3":
-            nbytes = max(self.ctx.lib.bsmps_fp16x3_row_plane_bytes(n, c) for c in self.chis)
-            pre_planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=self.dev)
-            rowscale = torch.empty((n,), dtype=torch.float32, device=self.dev)
-        for i in range(M):
-            last = i == M - 1
-            chi, d = self.chis[i], self.ds[i]
-            if i == 0:
-                src, stride = self.gammas[0], 0  # Gamma_0[0, a, j] is the site-0 tensor for every sample
-            else:
-                chi_p = self.chis[i - 1]
-                with timer("site_contract"):
-                    if self.precision == "fp64":
-                        ctx.call("bsmps_site_contract", n, chi_p, chi, d, runtime.ptr(pre),
-                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
-                    elif self.precision == "fp32":
-                        ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre),
-                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(self.gammas[i]), runtime.ptr(T))
-                    elif self.precision == "fp16x3":
-                        planes, colscale = self.splits[i]
-                        ctx.call("bsmps_split_rows_fp16x3", n, chi_p, runtime.ptr(pre), runtime.ptr(pre_planes),
-                                 runtime.ptr(rowscale))
-                        ctx.call("bsmps_site_contract_fp16x3", n, chi_p, chi, d, runtime.ptr(pre_planes),
-                                 runtime.ptr(rowscale), runtime.ptr(planes), runtime.ptr(colscale), runtime.ptr(T))
-                    elif self.precision == "bf16x6":
-                        ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre), runtime.ptr(pre_planes))
-                        ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(pre_planes),
-                                 runtime.ptr(self.splits[i]), runtime.ptr(T))
-                    else:
-                        hi, lo = self.splits[i]
-                        ctx.call("bsmps_site_contract_bf16x3", n, chi_p, chi, d, runtime.ptr(pre),
-                                 runtime.ptr(self.lambdas[i - 1]), runtime.ptr(hi), runtime.ptr(lo), runtime.ptr(T))
-                src, stride = T, chi * d
-            a_ptr = ctypes.c_void_p(alpha.data_ptr() + 16 * i)
-            s_ptr = ctypes.c_void_p(samples.data_ptr() + 4 * i)
-            p_ptr = runtime.ptr(probs[i]) if probs is not None else None
-            with timer("sample_site"):
-                ctx.call(draw, n, chi, d, dd, runtime.ptr(src), stride, a_ptr, M,
-                         runtime.ptr(self.W), runtime.ptr(self.x), runtime.ptr(dwork),
-                         None if last else runtime.ptr(self.lambdas[i]),
-                         runtime.ptr(uniforms[i]), s_ptr, M, None if last else runtime.ptr(pre), p_ptr)
-        return (samples, probs) if return_probs else samples
+        walk = _Walk(self.ctx, self.precision, self.M, self.dd, self.sqrtW, self.W, self.x, normals, self.chi,
+                     max(c * dk for c, dk in zip(self.chis, self.ds)), self.d, return_probs=return_probs)
+        for i in range(self.M):
+            walk.site(i, self.forms[i], self.lambdas[i - 1] if i else None,
+                      self.lambdas[i] if i < self.M - 1 else None, uniforms[i], timer)
+        return (walk.samples, walk.probs) if return_probs else walk.samples
+
+
+class SiteSampler:
+    """Site-major sampling with one site tensor on the device at a time, as the reference
+    loads Gamma_i per site inside its sampling loop (sampling_cpu.py:76-80): ``push(i, Gamma_i,
+    Lambda_{i-1}, Lambda_i)`` advances every sample through site i -- in chunks of ``chunk``
+    samples, so the contraction output stays small -- and then lets the site's device data go.
+    The carried vectors of all n samples stay on the device between sites.
+
+    ``chi_max`` / ``d_max`` bound the live ranks and cutoffs of the sites that will be pushed.
+    ``push`` takes the site's uniforms from ``uniforms`` unless a row is passed (the replay of
+    another form's sample paths, ``mid_uniforms_row``)."""
+
+    def __init__(self, M, dd, sqrtW, normals, uniforms, chi_max, d_max, precision="fp64", chunk=4096,
+                 return_probs=True, ctx=None):
+        if precision not in Sampler.PRECISIONS:
+            raise ValueError(f"precision must be one of {Sampler.PRECISIONS}")
+        if dd < d_max:
+            raise ValueError("dd must be >= d")
+        self.ctx = ctx or runtime.context()
+        self.M, self.dd, self.precision = M, dd, precision
+        self.chi_max, self.d_max = chi_max, d_max
+        self.uniforms = uniforms.contiguous() if uniforms is not None else None
+        self.sqrtW = runtime.to_device(np.ascontiguousarray(sqrtW, dtype=np.float64))
+        W, x = displacement_basis(dd)
+        self.W, self.x = runtime.to_device(W), runtime.to_device(x)
+        self.walk = _Walk(self.ctx, precision, M, dd, self.sqrtW, self.W, self.x, normals, chi_max,
+                          chi_max * d_max, d_max, chunk=chunk, return_probs=return_probs)
+        self.next_site = 0
+        self.rows = 1
+        self.lam_prev = None
+
+    def push(self, i, gamma, lam, uniforms_row=None, timer=None):
+        """Advance all samples through site i.  ``gamma`` is Gamma_i (device/host tensor or
+        NumPy array, at least [rank_{i-1}, rank_i, d]); ``lam`` is Lambda_i (ignored at the last
+        site).  Sites must come in order 0, 1, ..., M-1."""
+        import contextlib
+        if i != self.next_site:
+            raise ValueError(f"site {i} pushed, site {self.next_site} expected")
+        last = i == self.M - 1
+        if last:
+            cols = 1
+            lam_dev = None
+        else:
+            lam_h = np.asarray(runtime.host(lam), dtype=np.float64)
+            nz = np.flatnonzero(lam_h)
+            cols = max(1, min(int(nz[-1]) + 1 if len(nz) else 1, int(gamma.shape[1])))
+            lam_dev = runtime.to_device(lam_h)
+        if cols > self.chi_max or int(gamma.shape[2]) > self.d_max:
+            raise ValueError(f"site {i}: rank {cols} / cutoff {gamma.shape[2]} beyond chi_max {self.chi_max} / "
+                             f"d_max {self.d_max}")
+        if gamma.shape[0] < self.rows or gamma.shape[1] < cols:
+            raise ValueError(f"Gamma_{i} of shape {tuple(gamma.shape)} is smaller than its live block "
+                             f"[{self.rows}, {cols}]")
+        f = _prepare_site(self.ctx, self.precision, i, gamma, self.rows, cols, self.lam_prev)
+        u = uniforms_row if uniforms_row is not None else self.uniforms[i]
+        self.walk.site(i, f, self.lam_prev, lam_dev, u.contiguous(),
+                       timer or (lambda name: contextlib.nullcontext()))
+        self.rows, self.lam_prev = cols, lam_dev
+        self.next_site += 1
+
+    def state(self):
+        """Host copies of what continuing at the next site needs: the carried vectors
+        ([n][rank of the last cut walked]), the draws so far, and Lambda of that cut.  The
+        displacements are recomputed from the normals."""
+        w = self.walk
+        pre = w.pre[w.cur][:w.n * self.rows] if self.next_site > 0 else w.pre[w.cur][:0]
+        return {"next_site": np.array(self.next_site), "rows": np.array(self.rows),
+                "pre": runtime.to_host(pre), "samples": runtime.to_host(w.samples),
+                "lam_prev": runtime.to_host(self.lam_prev) if self.lam_prev is not None else np.zeros(0)}
+
+    def restore(self, st):
+        """Continue from ``state()`` of a sampler over the same normals and uniforms."""
+        w = self.walk
+        self.next_site = int(st["next_site"])
+        self.rows = int(st["rows"])
+        if self.rows > self.chi_max:
+            raise ValueError("checkpoint rank beyond chi_max")
+        w.samples.copy_(torch.from_numpy(np.ascontiguousarray(st["samples"])))
+        w.cur = 0
+        pre = np.ascontiguousarray(st["pre"])
+        if pre.size:
+            w.pre[0][:pre.size].copy_(torch.from_numpy(pre))
+        lam = np.asarray(st["lam_prev"], dtype=np.float64)
+        self.lam_prev = runtime.to_device(lam) if lam.size else None
+
+    @property
+    def samples(self):
+        return self.walk.samples
+
+    @property
+    def probs(self):
+        return self.walk.probs
+
+    def mid_uniforms_row(self, i):
+        """Uniforms [n] in the middle of each sample's cdf interval at site i (the draw is the
+        number of cdf entries below the uniform), so another form replaying site i with them
+        repeats these draws unless its probabilities differ by half an interval."""
+        if self.walk.probs is None:
+            raise ValueError("mid_uniforms_row needs return_probs=True")
+        p = self.walk.probs[i]
+        idx = self.walk.samples[:, i].long().unsqueeze(1)
+        cdf = torch.cumsum(p, dim=1)
+        return (cdf.gather(1, idx) - 0.5 * p.gather(1, idx)).squeeze(1).contiguous()
 
 
 def sampling(gammas, lambdas, sqrtW, dd, n, seed=0, offset=0):
@@ -261,16 +439,20 @@ def main(argv=None):
     cov = np.load(args.dir + "cov.npy")
     sqrtW = thermal_sqrtW(cov, sq_cov)
     M = sqrtW.shape[0] // 2
-    dev = runtime.context().device
-    gammas = [torch.from_numpy(np.load(path + f'Gamma_{i}.npy').astype(np.complex128)).to(dev) for i in range(M)]
-    lambdas = [np.load(path + f"Lambda_{i}.npy").astype(np.float64) for i in range(M - 1)] + [np.zeros(args.chi)]
-    sampler = Sampler(gammas, lambdas, sqrtW, args.dd)
+    # Lambda_i as float64 whatever dtype the files hold (the reference saves float32)
+    lambdas = [np.load(path + f"Lambda_{i}.npy").astype(np.float64) for i in range(M - 1)] + [None]
     for it in range(args.iter):
         chunks = []
         for b0 in range(0, args.N, args.n):
             nb = min(args.N, b0 + args.n) - b0
             normals, uniforms = random_inputs(M, nb, args.seed + it, b0)
-            chunks.append(runtime.to_host(sampler.run(normals, uniforms)).astype(np.int64))
+            walker = SiteSampler(M, args.dd, sqrtW, normals, uniforms, args.chi, args.d, chunk=nb,
+                                 return_probs=False)
+            for i in range(M):
+                # one site tensor at a time, as sampling_cpu.py:76 loads Gamma_i per site; the
+                # memory map reads only the live block (complex64 files are widened on the device)
+                walker.push(i, np.load(path + f'Gamma_{i}.npy', mmap_mode='r'), lambdas[i])
+            chunks.append(runtime.to_host(walker.samples).astype(np.int64))
             np.save(args.dir + f"samples_{it}.npy", np.concatenate(chunks, axis=0))
 
 
