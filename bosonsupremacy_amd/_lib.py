@@ -63,6 +63,9 @@ _PROTOTYPES = {
     "bsmps_gamma_site": ([c_void_p, ctypes.POINTER(GammaSiteArgs), c_void_p], c_int),
     "bsmps_kron_cuts": ([c_void_p, c_int, c_int32_p, c_int, c_int, c_void_p, c_int64, c_int, c_double,
                          c_void_p, c_void_p, c_void_p], c_int),
+    "bsmps_kron_cuts_noalloc": ([c_void_p, c_int, c_int32_p, c_int, c_int, c_void_p, c_int64, c_int, c_double,
+                                 c_void_p, c_void_p, c_void_p], c_int),
+    "bsmps_kron_noalloc_scratch_bytes": ([], c_int64),
     "bsmps_mu_to_alpha": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p], c_int),
     "bsmps_batch_displaces": ([c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p], c_int),
     "bsmps_site_contract": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],

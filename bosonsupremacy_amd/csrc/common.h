@@ -75,6 +75,9 @@ struct bsmps_ctx {
 };
 
 int bsmps_ctx_workspace(bsmps_ctx* ctx, size_t bytes, void** out);
+// Looks up (once per device, with the device current) the address of the allocation-free
+// entry point's scratch array (kron.hip).
+int bsmps_kron_noalloc_init(int device);
 int bsmps_ctx_scratch(bsmps_ctx* ctx, size_t bytes, void** out);
 // Asynchronous host -> device copy of a small table through pinned staging: returns once the
 // table is staged, without waiting for the stream.

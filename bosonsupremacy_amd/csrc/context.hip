@@ -89,6 +89,7 @@ extern "C" int bsmps_create(int device, void* stream, bsmps_ctx** out) {
         return BSMPS_ERR_INVALID;
     }
     BSMPS_HIP(hipSetDevice(device));
+    BSMPS_TRY(bsmps_kron_noalloc_init(device));  // the device's allocation-free scratch address
     bsmps_ctx* c = new bsmps_ctx();
     c->device = device;
     c->ws = nullptr;

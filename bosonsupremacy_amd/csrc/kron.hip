@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -32,14 +33,12 @@ struct CutDesc {
 // np.outer(res, tp) in kron_cpu.py:110.
 __device__ __forceinline__ double yval(const double* cur, double f, int64_t a) { return cur[a] * f; }
 
-__global__ __launch_bounds__(kThreads) void kron_cut_kernel(const CutDesc* __restrict__ cuts, int max_modes, int d,
-                                                            const double* __restrict__ tp_all, int64_t max_dim,
-                                                            int chi, double err_tol, double* __restrict__ res_out,
-                                                            int8_t* __restrict__ num_out, int32_t* __restrict__ len_out,
-                                                            double* __restrict__ vbuf, int32_t* __restrict__ pbuf,
-                                                            int32_t* __restrict__ bp_all) {
-    const int c = blockIdx.x;
-    const CutDesc cd = cuts[c];
+// Cut c of a launch (workgroup c), described by cd.
+__device__ __forceinline__ void kron_cut(const CutDesc cd, int c, int max_modes, int d,
+                                         const double* __restrict__ tp_all, int64_t max_dim, int chi,
+                                         double err_tol, double* __restrict__ res_out, int8_t* __restrict__ num_out,
+                                         int32_t* __restrict__ len_out, double* __restrict__ vbuf,
+                                         int32_t* __restrict__ pbuf, int32_t* __restrict__ bp_all) {
     const int n_modes = cd.n_modes;
     const double* tp = tp_all + (int64_t)c * max_modes * d;
     double* V[3] = {vbuf + (int64_t)c * 3 * max_dim, vbuf + (int64_t)c * 3 * max_dim + max_dim,
@@ -171,7 +170,131 @@ __global__ __launch_bounds__(kThreads) void kron_cut_kernel(const CutDesc* __res
     }
 }
 
+// Cut descriptors read from device memory (the allocating entry point uploads them).
+__global__ __launch_bounds__(kThreads) void kron_cut_kernel(const CutDesc* __restrict__ cuts, int max_modes, int d,
+                                                            const double* __restrict__ tp_all, int64_t max_dim,
+                                                            int chi, double err_tol, double* __restrict__ res_out,
+                                                            int8_t* __restrict__ num_out, int32_t* __restrict__ len_out,
+                                                            double* __restrict__ vbuf, int32_t* __restrict__ pbuf,
+                                                            int32_t* __restrict__ bp_all) {
+    kron_cut(cuts[blockIdx.x], blockIdx.x, max_modes, d, tp_all, max_dim, chi, err_tol, res_out, num_out, len_out,
+             vbuf, pbuf, bp_all);
+}
+
+// Cut descriptors passed by value in the kernel arguments (the allocation-free entry point:
+// no upload, no host staging buffer, nothing to synchronise on).
+constexpr int kArgCuts = 64;  // 64 x 16 bytes of kernel arguments
+struct CutDescArgs {
+    CutDesc c[kArgCuts];
+};
+__global__ __launch_bounds__(kThreads) void kron_cut_args_kernel(CutDescArgs cuts, int max_modes, int d,
+                                                                 const double* __restrict__ tp_all, int64_t max_dim,
+                                                                 int chi, double err_tol, double* __restrict__ res_out,
+                                                                 int8_t* __restrict__ num_out,
+                                                                 int32_t* __restrict__ len_out,
+                                                                 double* __restrict__ vbuf, int32_t* __restrict__ pbuf,
+                                                                 int32_t* __restrict__ bp_all) {
+    kron_cut(cuts.c[blockIdx.x], blockIdx.x, max_modes, d, tp_all, max_dim, chi, err_tol, res_out, num_out, len_out,
+             vbuf, pbuf, bp_all);
+}
+
+// Scratch of the allocation-free entry point: a file-scope device array (one copy per GPU,
+// held until the process ends).  Host code never passes its name to a kernel -- on the host
+// that name is a stand-in, not a device address -- but the per-device address that
+// hipGetSymbolAddress returns, looked up once per device when the device's first context is
+// created (bsmps_kron_noalloc_init, called by bsmps_create).
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+__device__ __align__(256) char kron_noalloc_scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Bytes of scratch cut c0..c1-1 take: value and payload buffers plus the back-pointer block.
+int64_t cut_fixed_bytes(int64_t max_dim) { return max_dim * (3 * 8 + 2 * 4); }
+int64_t cut_bp_words(int nm, int64_t max_dim) { return (int64_t)std::max(nm - 1, 1) * max_dim; }
+
 }  // namespace
+
+int bsmps_kron_noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        bsmps_set_error("bsmps_create: device %d beyond the %d the allocation-free scratch table holds", device,
+                        kMaxDevices);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return BSMPS_OK;
+    void* p = nullptr;
+    BSMPS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(kron_noalloc_scratch)));  // the current device's copy
+    g_scratch_ptr[device] = p;
+    return BSMPS_OK;
+}
+
+extern "C" int64_t bsmps_kron_noalloc_scratch_bytes(void) { return (int64_t)kNoAllocScratchBytes; }
+
+extern "C" int bsmps_kron_cuts_noalloc(bsmps_ctx* ctx, int n_cuts, const int32_t* n_modes_host, int max_modes,
+                                       int d, const double* tp, int64_t max_dim, int chi, double err_tol,
+                                       double* res, int8_t* num, int32_t* len) {
+    if (!ctx || n_cuts < 0 || !n_modes_host || max_modes <= 0 || d <= 0 || d > kMaxD || max_dim <= 0 ||
+        max_dim > (int64_t(1) << 26) || chi <= 0) {
+        bsmps_set_error("bsmps_kron_cuts_noalloc: invalid arguments (d must be 1..%d, max_dim 1..2^26)", kMaxD);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n_cuts == 0) return BSMPS_OK;
+    if (!tp || !res || !num || !len) {
+        bsmps_set_error("bsmps_kron_cuts_noalloc: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    char* base = ctx->device >= 0 && ctx->device < kMaxDevices ? static_cast<char*>(g_scratch_ptr[ctx->device])
+                                                               : nullptr;
+    if (!base) {
+        bsmps_set_error("bsmps_kron_cuts_noalloc: no scratch address for device %d", ctx->device);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+    const int64_t cap = (int64_t)kNoAllocScratchBytes;
+    // every cut must fit the scratch on its own; checked before anything is launched
+    for (int c = 0; c < n_cuts; ++c) {
+        const int nm = n_modes_host[c];
+        if (nm < 1 || nm > max_modes) {
+            bsmps_set_error("bsmps_kron_cuts_noalloc: cut %d has %d modes (max_modes %d)", c, nm, max_modes);
+            return BSMPS_ERR_INVALID;
+        }
+        const int64_t need = cut_fixed_bytes(max_dim) + cut_bp_words(nm, max_dim) * 4 + 1024;
+        if (need > cap) {
+            bsmps_set_error("bsmps_kron_cuts_noalloc: cut %d (%d modes, max_dim %lld) needs %lld bytes of scratch, "
+                            "more than the %lld this entry point has", c, nm, (long long)max_dim, (long long)need,
+                            (long long)cap);
+            return BSMPS_ERR_UNSUPPORTED;
+        }
+    }
+    // chunks of at most kArgCuts cuts whose buffers fit the scratch, one launch each, in order
+    // on the context's stream (the next chunk reuses the scratch after the previous has run)
+    int c0 = 0;
+    while (c0 < n_cuts) {
+        CutDescArgs args;
+        int64_t bp_words = 0;
+        int c1 = c0;
+        while (c1 < n_cuts && c1 - c0 < kArgCuts) {
+            const int64_t words = cut_bp_words(n_modes_host[c1], max_dim);
+            const int64_t nc = c1 - c0 + 1;
+            const int64_t bytes = ((nc * 3 * max_dim * 8 + 255) / 256) * 256 + ((nc * 2 * max_dim * 4 + 255) / 256) * 256 +
+                                  (bp_words + words) * 4;
+            if (c1 > c0 && bytes > cap) break;
+            args.c[c1 - c0] = CutDesc{n_modes_host[c1], 0, bp_words};
+            bp_words += words;
+            ++c1;
+        }
+        const int nc = c1 - c0;
+        const size_t bytes_v = ((size_t)nc * 3 * max_dim * sizeof(double) + 255) / 256 * 256;
+        const size_t bytes_p = ((size_t)nc * 2 * max_dim * sizeof(int32_t) + 255) / 256 * 256;
+        hipLaunchKernelGGL(kron_cut_args_kernel, dim3(nc), dim3(kThreads), 0, ctx->stream, args, max_modes, d,
+                           tp + (int64_t)c0 * max_modes * d, max_dim, chi, err_tol, res + (int64_t)c0 * chi,
+                           num + (int64_t)c0 * chi * max_modes, len + c0, reinterpret_cast<double*>(base),
+                           reinterpret_cast<int32_t*>(base + bytes_v), reinterpret_cast<int32_t*>(base + bytes_v + bytes_p));
+        BSMPS_HIP(hipGetLastError());
+        c0 = c1;
+    }
+    return BSMPS_OK;
+}
 
 extern "C" int bsmps_kron_cuts(bsmps_ctx* ctx, int n_cuts, const int32_t* n_modes_host, int max_modes, int d,
                                const double* tp, int64_t max_dim, int chi, double err_tol, double* res, int8_t* num,

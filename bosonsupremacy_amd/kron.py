@@ -34,11 +34,13 @@ def cut_spectrum_inputs(sq_cov, L, cutoff):
     return gaussian.cut_inputs(sq_cov, L, cutoff)
 
 
-def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None):
+def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None, no_alloc=False):
     """Top-chi Schmidt weights and labels of several cuts at once on the GPU.
 
     tps: list of arrays [n_c, d] (thermal distributions of each cut's modes).
     Returns a list of (res float64 [len], num int8 [len, n_c] -- 1-D when n_c == 1).
+    ``no_alloc`` runs the library's allocation-free entry point (fixed device scratch; the
+    same results).
     """
     ctx = runtime.context(device)
     n_cuts = len(tps)
@@ -56,7 +58,7 @@ def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None):
     num_d = torch.zeros((n_cuts, chi, max_modes), dtype=torch.int8, device=dev)
     len_d = torch.zeros((n_cuts,), dtype=torch.int32, device=dev)
     with (timer("kron_cuts") if timer else contextlib.nullcontext()):
-        ctx.call("bsmps_kron_cuts", n_cuts, n_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_modes, d,
+        ctx.call("bsmps_kron_cuts_noalloc" if no_alloc else "bsmps_kron_cuts", n_cuts, n_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_modes, d,
                  runtime.ptr(tp_d), int(max_dim), int(chi), float(err_tol), runtime.ptr(res_d),
                  runtime.ptr(num_d), runtime.ptr(len_d))
     res_h, num_h, len_h = runtime.to_host(res_d), runtime.to_host(num_d), runtime.to_host(len_d)

@@ -147,6 +147,22 @@ int bsmps_kron_cuts(bsmps_ctx *ctx, int n_cuts, const int32_t *n_modes_host, int
                     const double *tp, int64_t max_dim, int chi, double err_tol, double *res,
                     int8_t *num, int32_t *len);
 
+/* Allocation-free variant of bsmps_kron_cuts.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results bit-identical to
+ * bsmps_kron_cuts.  Its scratch is a fixed device array of bsmps_kron_noalloc_scratch_bytes()
+ * bytes (256 MiB) per GPU, held until the process ends; it calls no hipMalloc/hipFree, copies
+ * nothing from the host and does not synchronise (cut descriptors travel in the kernel
+ * arguments, cuts run in launches of at most 64 that fit the scratch).  A cut needing more
+ * scratch than that -- max_dim * (32 + 4 * (n_modes - 1)) bytes, about 1300 modes at
+ * max_dim = 1e5 -- is refused with BSMPS_ERR_UNSUPPORTED before anything is launched.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race. */
+int bsmps_kron_cuts_noalloc(bsmps_ctx *ctx, int n_cuts, const int32_t *n_modes_host, int max_modes,
+                            int d, const double *tp, int64_t max_dim, int chi, double err_tol,
+                            double *res, int8_t *num, int32_t *len);
+int64_t bsmps_kron_noalloc_scratch_bytes(void);
+
 /* ---- sampling (L4) --------------------------------------------------------------------- */
 
 /* alpha[s][i] = (mu[s][i] + 1j mu[s][M+i]) / 2 with mu = sqrtW @ normals[:, s]
