@@ -315,17 +315,40 @@ def partial_run(args):
     print(json.dumps(out))
 
 
-def site_hafnian_work(kres, knum, d, err_tol=1e-10):
-    """(hafnian count, flops) of all site tensors, from the label photon counts."""
+def survey_hafnian_flops(size):
+    """Algorithmic FP64 flops of one hafnian of the given size in the reference recursion
+    (MPS_cpu.py:117-187) with every loop run in full: F(2n) = 8 C(n) + 8 2^n - 2 with
+    C(n) = sum_{l<n} 2^l n(n+1)/2 (1 + (2n-2l-2)(2n-2l-3)); F(4) = 22 (three products and two
+    adds), F(2) = F(0) = 0, odd sizes 0 (a complex product counts 6, an add 2, a multiply-add 8)."""
+    if size % 2 or size <= 2:
+        return 0
+    if size == 4:
+        return 22
+    n = size // 2
+    C = sum(2 ** l * n * (n + 1) // 2 * (1 + (2 * n - 2 * l - 2) * (2 * n - 2 * l - 3)) for l in range(n))
+    return 8 * C + 8 * 2 ** n - 2
+
+
+def site_hafnian_work(kres, knum, d, err_tol=1e-10, pruned=False):
+    """(hafnian count, executed flops, algorithmic flops) of all site tensors, from the label
+    photon counts.  Executed flops skip structurally zero and dead terms (hafnian_flops);
+    algorithmic flops are the reference recursion's full loops (survey_hafnian_flops).
+    ``pruned``: only the kept labels of cut k-1 enter site k (mps.build_mps(pruned=True))."""
     M = len(kres) + 1
-    total_n, total_f = 0, 0.0
+    total_n, total_f, total_a = 0, 0.0, 0.0
     fl = {}
     for k in range(M):
         if k == M - 1:
             la = np.zeros(1, int)
         else:
             la = np.asarray(knum[k]).reshape(len(kres[k]), -1)[np.asarray(kres[k]) > err_tol].sum(1)
-        lb = np.zeros(1, int) if k == 0 else np.asarray(knum[k - 1]).reshape(len(knum[k - 1]), -1).sum(1)
+        if k == 0:
+            lb = np.zeros(1, int)
+        else:
+            lb = np.asarray(knum[k - 1]).reshape(len(knum[k - 1]), -1)
+            if pruned:
+                lb = lb[np.asarray(kres[k - 1]) > err_tol]
+            lb = lb.sum(1)
         ca, cb = np.bincount(la), np.bincount(lb)
         for j in range(d):
             for p in range(len(ca)):
@@ -335,29 +358,33 @@ def site_hafnian_work(kres, knum, d, err_tol=1e-10):
                         continue
                     cnt = int(ca[p]) * int(cb[q])
                     if n // 2 not in fl:
-                        fl[n // 2] = hafnian_flops(n // 2)
+                        fl[n // 2] = (hafnian_flops(n // 2), survey_hafnian_flops(n))
                     total_n += cnt
-                    total_f += cnt * fl[n // 2]
-    return total_n, total_f
+                    total_f += cnt * fl[n // 2][0]
+                    total_a += cnt * fl[n // 2][1]
+    return total_n, total_f, total_a
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload, n_samples):
-    """HBM bytes per site_contract launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    runs (profiles/<workload>_traffic.json, scaled by a 1 GiB copy), when they cover this
-    workload and sample count; None otherwise."""
+def profiled_traffic(workload, kernel):
+    """HBM bytes per call of ``kernel`` on ``workload`` from a committed rocprofv3 summary
+    (profiles/<workload>_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same
+    counters on a copy of known size), or None.  Reported as coming from that profile, not
+    measured in this run (counters need their own rocprofv3 pass)."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("n_samples") != n_samples:
+    entry = t.get("kernels", {}).get(kernel)
+    if entry is None:
         return None
-    return 1024.0 * (t["fetch_kib_per_dispatch"] * t["fetch_factor"] + t["write_kib_per_dispatch"] * t["write_factor"])
+    return {"bytes_per_call": entry["bytes_per_call"], "source": os.path.relpath(path, ROOT),
+            "command": t.get("command")}
 
 
 class KernelTimer:
@@ -393,46 +420,159 @@ class KernelTimer:
         return out
 
 
-def cpu_baseline(inst, kron_out, seconds=20.0):
-    """The FP64 oracle (NumPy, single process) on a bounded sample of the workload:
-    site tensors for a random subset of one mid-chain site's left labels (all right labels,
-    all j), scaled to whole sites; and sampling of a small batch."""
+def host_record():
+    """The host the CPU baseline ran on."""
+    import platform
+    rec = {"cpu_count": os.cpu_count(), "numpy": np.__version__, "machine": platform.machine()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    rec["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        rec["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        from threadpoolctl import threadpool_info
+        rec["blas"] = [{"api": i.get("internal_api"), "threads": i.get("num_threads")} for i in threadpool_info()
+                       if i.get("user_api") == "blas"]
+    except Exception:
+        pass
+    return rec
+
+
+def _oracle_site_rows(args):
+    """Worker: the oracle's site k restricted to the given left labels (single-threaded BLAS)."""
     from oracle import gbs_oracle as O
-    kres, knum, kS = kron_out
-    M, d, chi = inst.M, inst.d, inst.chi
-    k = M // 2
+    k, M, sq_cov, kres, knum, kS, d, chi, rows = args
     res = np.asarray(kres[k])
-    nL = int((res > 1e-10).sum())
-    rng = np.random.default_rng(0)
-    n_right = len(knum[k - 1])
-    frac_rows = max(1, min(int(0.01 * nL), int(1e5 // (n_right * d))))  # ~10^5 oracle hafnians
-    rows = np.sort(rng.choice(nL, size=frac_rows, replace=False))
-    sub_res = list(kres)
-    sub_num = list(knum)
+    sub_res, sub_num = list(kres), list(knum)
     sub_res[k] = res[rows]
     sub_num[k] = np.asarray(knum[k]).reshape(len(res), -1)[rows]
     t0 = time.perf_counter()
-    O.build_site(k, M, inst.sq_cov, sub_res, sub_num, kS, d, chi)
-    t_site = (time.perf_counter() - t0) * nL / frac_rows
-    sites_per_s = 1.0 / t_site
-    return {
-        "value": sites_per_s, "unit": "sites/s", "cores": 1, "kind": "port",
-        "sample": f"oracle build_site on {frac_rows}/{nL} left labels of site {k} "
-                  f"(all {len(knum[k - 1])} right labels, all j), scaled to a whole site",
-    }
+    O.build_site(k, M, sq_cov, sub_res, sub_num, kS, d, chi)
+    return time.perf_counter() - t0
 
 
-def cpu_sampling_rate(inst, G, L, n=50):
+def cpu_pool(workers=16):
+    """Worker processes for the all-cores CPU baseline, forked before this process initialises
+    the GPU (they only ever run NumPy), started at once so the fork happens now."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    n = max(1, min(workers, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else workers))
+    ex = ProcessPoolExecutor(max_workers=n, mp_context=mp.get_context("fork"), initializer=_cpu_worker_init)
+    for f in [ex.submit(os.getpid) for _ in range(n)]:
+        f.result()
+    return ex, n
+
+
+def _cpu_worker_init():
+    """One BLAS thread per worker: the processes themselves use the cores."""
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:
+        pass
+
+
+def cpu_baseline(inst, kron_out, G, L, pool):
+    """The FP64 oracle (this repository's NumPy restatement of the reference's CPU scripts) on
+    bounded samples of the workload, in the two modes of the reference's CPU path:
+
+    * one process, as MPS_cpu.py / sampling_cpu.py run (elementwise NumPy on one core): one
+      mid-chain site restricted to a random subset of its left labels (all right labels, all j),
+      scaled to whole sites, and 40 full samples;
+    * all host cores: ``workers`` processes (at most the cores this process may use), each on
+      its own subset of the same site's left labels with single-threaded BLAS, their union scaled
+      to whole sites (sites are independent given the spectra, so this is how the CPU path
+      parallelises)."""
     from oracle import gbs_oracle as O
     from bosonsupremacy_amd import runtime, sampling
+    ex, pool_n = pool
+    kres, knum, kS = kron_out
+    M, d, chi = inst.M, inst.d, inst.chi
+    k = M // 2
+    nL = int((np.asarray(kres[k]) > 1e-10).sum())
+    n_right = int((np.asarray(kres[k - 1]) > 1e-10).sum())
+    per = max(1, min(nL // 20, int(3e5 // max(1, n_right * d))))  # ~3e5 oracle hafnian entries per worker
+    rng = np.random.default_rng(0)
+    picks = rng.permutation(nL)[:per * (pool_n + 1)]
+    one = np.sort(picks[:per])
+    t_one = _oracle_site_rows((k, M, inst.sq_cov, kres, knum, kS, d, chi, one))
+    out = {"value": per / nL / t_one, "unit": "sites/s", "cores": 1, "kind": "port",
+           "sample": f"oracle site {k} on {per} of its {nL} left labels (all right labels, all j), one "
+                     f"process, scaled to a whole site", "host": host_record()}
+    chunks = [np.sort(picks[per * (i + 1):per * (i + 2)]) for i in range(pool_n)]
+    t0 = time.perf_counter()
+    list(ex.map(_oracle_site_rows, [(k, M, inst.sq_cov, kres, knum, kS, d, chi, c) for c in chunks]))
+    wall = time.perf_counter() - t0
+    out["all_cores"] = {"value": per * pool_n / nL / wall, "unit": "sites/s", "cores": pool_n,
+                        "sample": f"{pool_n} processes (one BLAS thread each) x {per} left labels of site {k}, "
+                                  f"wall time of the pool, scaled to a whole site"}
+    # sampling: one process (the reference's sampling_cpu.py loop)
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     oG = [runtime.to_host(g) for g in G]
-    rng = np.random.default_rng(0)
-    normals = rng.normal(size=(2 * inst.M, n))
-    uniforms = rng.uniform(size=(inst.M, n))
+    chi_max = max(g.shape[1] for g in oG)
+    padded = []
+    for g in oG:   # the oracle walks [chi, chi, d] tensors
+        p = np.zeros((chi_max, chi_max, g.shape[2]), dtype=np.complex128)
+        p[:g.shape[0], :g.shape[1]] = g
+        padded.append(p)
+    lam = np.zeros((chi_max, M - 1))
+    for i in range(M - 1):
+        lam[:len(L[i]), i] = L[i]
+    n = 40
+    normals = rng.normal(size=(2 * M, n))
+    uniforms = rng.uniform(size=(M, n))
     t0 = time.perf_counter()
-    O.sampling(oG, np.stack(L[:-1], axis=1), sqrtW, inst.dd, normals, uniforms)
-    return n / (time.perf_counter() - t0)
+    O.sampling(padded, lam, sqrtW, inst.dd, normals, uniforms)
+    out["samples_per_s"] = n / (time.perf_counter() - t0)
+    out["samples_sample"] = f"oracle sampling of {n} full samples, one process"
+    return out
+
+
+def m144_probe(n_samples=4096):
+    """The M = 144, chi = 10^4 workload at full size where one session allows it: the spectra of
+    all 143 cuts, the exact hafnian work of every site (pruned plain form), the middle site built
+    at full size, and one sampling step through it of ``n_samples`` samples in FP64 and FP16x3.
+    Chain figures are projections from these measurements and are labelled so."""
+    import torch
+    from bosonsupremacy_amd import instances, kron, mps, runtime
+    inst = instances.workload("m144_d10_chi10000")
+    t0 = time.perf_counter()
+    ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+    torch.cuda.synchronize()
+    t_kron = time.perf_counter() - t0
+    k = inst.M // 2
+    t0 = time.perf_counter()
+    G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, sites=[k], pruned=True)
+    torch.cuda.synchronize()
+    t_site = time.perf_counter() - t0
+    work = site_hafnian_work_by_site(ko[0], ko[1], inst.d,
+                                     site_chi=[int((np.asarray(r) > 1e-10).sum()) for r in ko[0]])
+    rate = work[k][1] / t_site
+    probe = probe_sampling(inst, ko, (k, G[k], L[k]), n_samples, {})
+    del G
+    total_flops = sum(w[1] for w in work)
+    proj_construction = t_kron + total_flops / rate
+    return {
+        "workload": "m144_d10_chi10000 (M=144, d=10, chi=10^4, dd=20)",
+        "spectra_s": t_kron,
+        "middle_site": {"site": k, "seconds": t_site, "hafnians": int(work[k][0]), "executed_flops": work[k][1],
+                        "largest_size": int(work[k][2]), "executed_tflops": rate / 1e12},
+        "sampling_step": probe,
+        "extrapolated": {
+            "note": "projections from the measured middle site and sampling step, not a chain run; the "
+                    "hafnian-heavy sites near the chain ends run at a higher rate than the middle site",
+            "construction_s_at_middle_site_rate": proj_construction,
+            "samples_per_s_fp64": probe["chain_samples_per_s_fp64"],
+            "samples_per_s_fp16x3": probe["chain_samples_per_s_fp16x3"]},
+        "hafnian_flops_all_sites": total_flops,
+    }
 
 
 def main():
@@ -440,10 +580,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="m32_d6_chi512")
+    ap.add_argument("--workload", default="m64_d8_chi2000")
     ap.add_argument("--samples", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-cutoff form")
+    ap.add_argument("--no-m144-probe", action="store_true", help="skip the M=144, chi=10^4 full-size leg")
     ap.add_argument("--no-forms", action="store_true",
                     help="time the plain step only (no precision / learned forms, no CPU baseline)")
     ap.add_argument("--partial-sites", default=None,
@@ -477,6 +618,7 @@ def main():
     if args.no_forms:
         args.no_learned = True
         args.no_cpu_baseline = True
+        args.no_m144_probe = True
     if args.partial_sites:
         partial_run(args)
         return
@@ -500,22 +642,39 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on a "
               f"single GPU with no collectives)", file=sys.stderr)
         sys.exit(2)
+    standard_run(args)
 
+
+# samples per step of each workload in the default bench (one batch of the workload's samples;
+# the M = 64 step with all 10^5 samples would take 30 s x 25 steps)
+SAMPLES_PER_STEP = {"m64_d8_chi2000": 10 ** 4}
+MARGINAL_BAR = 1e-6
+
+
+def form_flops(s, n):
+    """Complex multiply-adds of one pass's site contractions over the live ranks of sampler s."""
+    return sum(n * s.rows[i] * s.chis[i] * s.ds[i] for i in range(1, s.M))
+
+
+def standard_run(args):
     import torch
-    from bosonsupremacy_amd import batched, instances, kron, mps, runtime, sampling
+    from bosonsupremacy_amd import autotune, instances, kron, metrics, mps, runtime, sampling
 
+    t_begin = time.perf_counter()
+    pool = None if args.no_cpu_baseline else cpu_pool()   # forked before the GPU is initialised
     inst = instances.workload(args.workload)
-    n_samples = args.samples or inst.n_samples
-    ctx = runtime.context()
-    from bosonsupremacy_amd import autotune
+    n_samples = args.samples or SAMPLES_PER_STEP.get(args.workload, inst.n_samples)
+    runtime.context()
     tuning = autotune.load_or_tune()
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     timer = KernelTimer()
+    state = {}
 
     def step():
         t0 = time.perf_counter()
-        # spectra of every cut + every site tensor (host FP64 work pipelined with the GPU)
-        G, L, kron_out = mps.build_mps(inst.sq_cov, inst.d, inst.chi, timer=timer)
+        # spectra of every cut + every site tensor over the kept labels (pruned plain form; host
+        # FP64 work pipelined with the GPU)
+        G, L, kron_out = mps.build_mps(inst.sq_cov, inst.d, inst.chi, timer=timer, pruned=True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         s = sampling.Sampler(G, L, sqrtW, inst.dd)
@@ -525,7 +684,8 @@ def main():
         samples = s.run(normals, uniforms, timer=timer)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
-        return (t1 - t0, t3 - t2, t3 - t0), (kron_out, G, L, samples)
+        state.update(kron_out=kron_out, G=G, L=L, s=s)
+        return t1 - t0, t3 - t2
 
     for w in range(args.warmup):
         step()
@@ -533,65 +693,73 @@ def main():
     torch.cuda.synchronize()
     timer.enabled = True
     times = []
-    last = None
     t_start = time.perf_counter()
     for i in range(args.steps):
-        tt, last = step()
-        times.append(tt)
-        log(f"timed step {i + 1}: construction {tt[0]:.3f} s, sampling {tt[1]:.3f} s")
+        times.append(step())
+        log(f"timed step {i + 1}: construction {times[-1][0]:.3f} s, sampling {times[-1][1]:.3f} s")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    timer.enabled = False
     t_build = float(np.mean([t[0] for t in times]))
     t_samp = float(np.mean([t[1] for t in times]))
     kern = timer.summary()
+    kron_out, G, L, s0 = state["kron_out"], state["G"], state["L"], state["s"]
 
-    kron_out, G, L, samples = last
-    n_haf, haf_flops = site_hafnian_work(kron_out[0], kron_out[1], inst.d)
-    chi, d = inst.chi, inst.d
-    # per site_contract launch: MFMA flops of the contraction form in use (2x2 real blocks: 8 per
-    # complex multiply-add; Gauss: three real products, 6)
-    form64 = tuning.get("contract_fp64", "blocks")
-    gemm_flops = (6.0 if form64 == "gauss" else 8.0) * n_samples * chi * chi * d
+    # ---- per-kernel rates from the HIP events of the timed steps -----------------------------
+    n_haf, haf_exec, haf_alg = site_hafnian_work(kron_out[0], kron_out[1], inst.d, pruned=True)
+    cmacs = form_flops(s0, n_samples)
+    draw_bytes = sum(n_samples * (16 * s0.chis[i] * s0.ds[i] + (16 * s0.chis[i] if i < inst.M - 1 else 0)
+                                  + 8 + 16 + 4) for i in range(inst.M))
     kernels = {}
     for name, ds in kern.items():
         per_step = len(ds) / args.steps
         mean = float(np.mean(ds))
-        entry = {"launches_per_step": per_step, "mean_ms": mean * 1e3, "total_ms_per_step": mean * per_step * 1e3}
+        tot = mean * per_step
+        e = {"launches_per_step": per_step, "mean_ms": mean * 1e3, "total_ms_per_step": tot * 1e3}
         if name == "site_contract":
-            entry["tflops"] = gemm_flops / mean / 1e12
+            e["tflops"] = 8 * cmacs / tot / 1e12
+            e["flops_per_step"] = 8.0 * cmacs
         if name == "gamma_site":
-            entry["tflops"] = haf_flops / (mean * per_step) / 1e12
-            entry["hafnians_per_step"] = n_haf
+            e.update({"hafnians_per_step": n_haf, "algorithmic_flops_per_step": haf_alg,
+                      "executed_flops_per_step": haf_exec, "tflops": haf_alg / tot / 1e12,
+                      "executed_tflops": haf_exec / tot / 1e12})
         if name == "sample_site":
-            # algorithmic bytes per (sample, site): read T (chi d complex), write pre (chi complex),
-            # read a uniform and alpha, write the draw
-            byts = n_samples * (chi * d * 16 + chi * 16 + 8 + 16 + 4)
-            entry["gbs"] = byts / mean / 1e9
-        kernels[name] = entry
+            e["gbs"] = draw_bytes / tot / 1e9
+            e["algorithmic_bytes_per_step"] = draw_bytes
+        kernels[name] = e
     dom = max(kernels, key=lambda k: kernels[k]["total_ms_per_step"])
-    if dom == "site_contract":
+    if dom == "gamma_site":
         ach = kernels[dom]["tflops"]
-        roof = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(args.workload, n_samples),
-                "algorithmic": (f"{'6' if form64 == 'gauss' else '8'}*n*chi^2*d = {gemm_flops:.3e} FP64 MFMA "
-                                f"flops per launch (contraction form: {form64})"),
-                "algorithmic_bytes": 16.0 * (n_samples * chi + chi * chi * d + n_samples * chi * d)}
-    elif dom == "gamma_site":
+        roof = {"kernel": "gamma_site (hafnians)", "bound": "fp64-vector", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
+                "algorithmic": "sum over hafnians of F(size), F(2n) = 8 C(n) + 8 2^n - 2 (reference recursion, "
+                               f"full loops): {haf_alg:.4e} flops per step in {kernels[dom]['launches_per_step']:.0f} "
+                               "calls",
+                "executed_tflops": kernels[dom]["executed_tflops"],
+                "note": "bit-exact NumPy complex arithmetic (separate product, FMA and add) caps FP64 vector use at "
+                        "8/12 of peak; executed flops skip structurally zero and dead polynomial terms"}
+        tr = profiled_traffic(args.workload, "gamma_site")
+    elif dom == "site_contract":
         ach = kernels[dom]["tflops"]
-        roof = {"kernel": dom, "bound": "fp64-vector", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+        roof = {"kernel": "site_contract", "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
+                "algorithmic": f"8 n chi_(k-1) chi_k d over the live ranks: {8 * cmacs:.4e} flops per step"}
+        tr = profiled_traffic(args.workload, "site_contract")
     else:
         ach = kernels[dom].get("gbs", 0.0)
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None}
+                "frac": ach / PEAK_HBM_GBS}
+        tr = profiled_traffic(args.workload, dom)
+    roof["traffic"] = tr["bytes_per_call"] if tr else None
+    if tr:
+        roof["traffic_source"] = f"{tr['source']} (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE per call, not this run)"
+    log(f"timed steps done at {time.perf_counter() - t_begin:.0f} s")
 
-    # precision forms on common random numbers: throughput and first-order marginal error
+    # ---- precision forms along the FP64 sample paths -----------------------------------------
     forms = {}
     normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=2)
-    rb = {}
-    from bosonsupremacy_amd import metrics
-    pairs = [(i, i + 1) for i in range(inst.M - 1)] + [(0, inst.M - 1), (inst.M // 4, 3 * inst.M // 4)]
     ref = {}
+    pairs = [(i, i + 1) for i in range(inst.M - 1)] + [(0, inst.M - 1), (inst.M // 4, 3 * inst.M // 4)]
     for prec in (() if args.no_forms else ("fp64", "fp32", "bf16x3", "bf16x6", "fp16x3")):
         s = sampling.Sampler(G, L, sqrtW, inst.dd, precision=prec)
         s.run(normals, uniforms)
@@ -605,114 +773,70 @@ def main():
         s.run(normals, uniforms, timer=ft)
         t_contract = sum(ft.summary().get("site_contract", [0.0]))
         smp, probs = s.run(normals, uniforms, return_probs=True)
-        if prec != "fp64":
-            # along the FP64 sample paths: each uniform moved to the middle of the FP64 draw's
-            # cdf interval, so a draw that flips at a cdf boundary (sampling, not arithmetic)
-            # does not send the two forms down different paths
-            draws_differing = int((smp != ref["samples"]).any(dim=1).sum())
-            smp, probs = s.run(normals, ref["mid"], return_probs=True)
-            off_path = int((smp != ref["samples"]).any(dim=1).sum())
-        rb[prec] = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
-        e2 = metrics.rb_second_order(smp, probs)
-        # complex multiply-adds of all site contractions; MFMA flops executed per one of them:
-        # 8 (FP64 / FP32, 2x2 real blocks), 3 x 8 (BF16x3, FP16x3), 6 x 8 (BF16x6)
-        cmacs = sum(n_samples * s.chis[i - 1] * s.chis[i] * s.ds[i] for i in range(1, inst.M))
         per, peak = {"fp64": (8, PEAK_FP64_TFLOPS), "fp32": (8, PEAK_FP32_MFMA_TFLOPS),
                      "bf16x3": (24, PEAK_BF16_MFMA_TFLOPS), "bf16x6": (48, PEAK_BF16_MFMA_TFLOPS),
                      "fp16x3": (24, PEAK_BF16_MFMA_TFLOPS)}[prec]  # dense FP16 MFMA peak = BF16
-        forms[prec] = {"samples_per_s": n_samples / dt, "lambda_spectrum_rel_err_vs_fp64": 0.0,
-                       "contract_ms_per_pass": t_contract * 1e3,
-                       "contract_complex_gemm_tflops": 8 * cmacs / t_contract / 1e12,
-                       "contract_mfma_utilisation": per * cmacs / t_contract / 1e12 / peak}
-        log(f"sampling form {prec}: {n_samples / dt:.0f} samples/s")
+        f = {"samples_per_s": n_samples / dt, "contract_ms_per_pass": t_contract * 1e3,
+             "contract_complex_gemm_tflops": 8 * cmacs / t_contract / 1e12,
+             "contract_mfma_utilisation": per * cmacs / t_contract / 1e12 / peak}
         if prec == "fp64":
-            ref = {"e2": e2, "samples": smp, "mid": batched.mid_uniforms(smp, probs)}
-            # the FP64 MPS against the exact lossy Gaussian state (chi/d truncation + sampling noise)
-            mean_x, cov_x = metrics.exact_photon_moments(inst.cov)
-            e1 = rb[prec].cpu().numpy()
-            cov_rb = e2 - np.triu(np.outer(e1, e1), 1)
-            iu = np.triu_indices(inst.M, 1)
-            forms[prec]["first_order_marginal_max_rel_err_vs_exact"] = float(np.max(np.abs(e1 - mean_x) / mean_x))
-            forms[prec]["photon_covariance_max_abs_err_vs_exact"] = float(np.max(np.abs(cov_rb[iu] - cov_x[iu])))
-            forms[prec]["photon_covariance_max_abs_exact"] = float(np.max(np.abs(cov_x[iu])))
+            e1 = metrics.rb_first_order(probs)
+            ref = {"e1": e1, "e2": metrics.rb_second_order(smp, probs), "samples": smp,
+                   "mid": sampling_mid(smp, probs)}
+            # the FP64 MPS against the ideal lossy Gaussian state (the analysis notebooks' figures)
+            smp_h = runtime.to_host(smp)
+            f["vs_ideal_photon_numbers"] = {
+                "first_order": metrics.correlation_summary(metrics.exact_photon_moments(inst.cov)[0], e1),
+                "second_order": metrics.correlation_summary(metrics.ideal_photon_correlations(inst.cov)[1],
+                                                            metrics.sample_photon_correlations(smp_h)[1])}
+            f["vs_ideal_clicks"] = {
+                "first_order": metrics.correlation_summary(metrics.ideal_click_correlations(inst.cov)[0],
+                                                           metrics.sample_click_correlations(smp_h)[0]),
+                "second_order": metrics.correlation_summary(metrics.ideal_click_correlations(inst.cov)[1],
+                                                            metrics.sample_click_correlations(smp_h)[1])}
         else:
-            rel = torch.abs(rb[prec] - rb["fp64"]) / torch.abs(rb["fp64"])
-            forms[prec]["first_order_marginal_max_rel_err_vs_fp64"] = float(rel.max())
+            draws_differing = int((smp != ref["samples"]).any(dim=1).sum())
+            smp, probs = s.run(normals, ref["mid"], return_probs=True)
+            e1 = metrics.rb_first_order(probs)
+            e2 = metrics.rb_second_order(smp, probs)
             big = np.abs(ref["e2"]) > 1e-3 * np.abs(ref["e2"]).max()
-            forms[prec]["second_order_marginal_max_rel_err_vs_fp64"] = float(
-                np.max(np.abs(e2[big] - ref["e2"][big]) / np.abs(ref["e2"][big])))
-            forms[prec]["pair_tvd_vs_fp64"] = metrics.pair_tvd(smp, ref["samples"], inst.dd, pairs)
-            forms[prec]["draws_differing_same_uniforms"] = draws_differing
-            forms[prec]["off_fp64_path"] = off_path
-            forms[prec]["speedup_vs_fp64"] = forms[prec]["samples_per_s"] / forms["fp64"]["samples_per_s"]
-        del probs, smp
+            f.update({
+                "first_order_marginal_max_rel_err_vs_fp64": float(np.max(np.abs(e1 - ref["e1"]) / np.abs(ref["e1"]))),
+                "second_order_marginal_max_rel_err_vs_fp64": float(np.max(np.abs(e2[big] - ref["e2"][big])
+                                                                          / np.abs(ref["e2"][big]))),
+                "pair_tvd_vs_fp64": metrics.pair_tvd(smp, ref["samples"], inst.dd, pairs),
+                "draws_differing_same_uniforms": draws_differing,
+                "off_fp64_path": int((smp != ref["samples"]).any(dim=1).sum()),
+                "speedup_vs_fp64": (n_samples / dt) / forms["fp64"]["samples_per_s"]})
+            f["within_marginal_bar"] = bool(max(f["first_order_marginal_max_rel_err_vs_fp64"],
+                                                f["second_order_marginal_max_rel_err_vs_fp64"]) <= MARGINAL_BAR)
+        forms[prec] = f
+        log(f"sampling form {prec}: {n_samples / dt:.0f} samples/s")
+        del probs, smp, s
 
-    # learned-cutoff form (per-site cutoffs and ranks from the MLP, each checked exactly) with
-    # the fastest reduced-precision contraction whose first-order marginals stayed within 1e-7 of
-    # FP64 above (a tenth of the 1e-6 bar), against the plain FP64 form on the same random numbers
+    # ---- learned-cutoff form against the pruned plain FP64 form --------------------------------
     learned_form = None
     if not args.no_learned:
-        from bosonsupremacy_amd import learned
-        t0 = time.perf_counter()
-        model = learned.load_or_train()
-        t_train = time.perf_counter() - t0
-        log(f"learned form: network ready in {t_train:.1f} s")
-
-        def learned_construction():
-            t0 = time.perf_counter()
-            ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
-            sd, sc, rep = learned.plan_sites(model, inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
-            G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=sd, site_chi=sc)
-            torch.cuda.synchronize()
-            return time.perf_counter() - t0, G2, L2, rep
-
-        learned_construction()
-        t_lc, G2, L2, rep = learned_construction()
-        accurate = [p for p in ("fp32", "bf16x6", "fp16x3")
-                    if p in forms and forms[p]["first_order_marginal_max_rel_err_vs_fp64"] <= 1e-7]
-        lprec = max(accurate, key=lambda p: forms[p]["samples_per_s"]) if accurate else "fp32"
-        s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision=lprec)
-        s2.run(normals, uniforms)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        s2.run(normals, uniforms)
-        torch.cuda.synchronize()
-        t_ls = time.perf_counter() - t0
-        smp, probs = s2.run(normals, uniforms, return_probs=True)
-        rb_l = (probs * torch.arange(inst.dd, device=probs.device, dtype=probs.dtype)).sum(dim=2).mean(dim=1)
-        e2 = metrics.rb_second_order(smp, probs)
-        big = np.abs(ref["e2"]) > 1e-3 * np.abs(ref["e2"]).max()
-        rel2 = float(np.max(np.abs(e2[big] - ref["e2"][big]) / np.abs(ref["e2"][big])))
-        tvd = metrics.pair_tvd(smp, ref["samples"], inst.dd, pairs)
-        del probs, smp
-        rel = float((torch.abs(rb_l - rb["fp64"]) / torch.abs(rb["fp64"])).max())
-        t_plain_samp = n_samples / forms["fp64"]["samples_per_s"]
-        learned_form = dict(rep)
-        learned_form.update({
-            "precision": f"{lprec} contraction",
-            "training_s": t_train,
-            "construction_s": t_lc,
-            "construction_speedup_vs_plain": t_build / t_lc,
-            "samples_per_s": n_samples / t_ls,
-            "sampling_speedup_vs_plain_fp64": t_plain_samp / t_ls,
-            "end_to_end_speedup_vs_plain_fp64": (t_build + t_plain_samp) / (t_lc + t_ls),
-            "first_order_marginal_max_rel_err_vs_plain_fp64": rel,
-            "second_order_marginal_max_rel_err_vs_plain_fp64": rel2,
-            "pair_tvd_vs_plain_fp64": tvd,
-        })
-        del G2, L2, s2
+        learned_form = learned_leg(inst, sqrtW, n_samples, normals, ref, forms, t_build, t_samp)
+    log(f"forms done at {time.perf_counter() - t_begin:.0f} s")
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline")
-        cpu = cpu_baseline(inst, kron_out)
-        cpu["samples_per_s"] = cpu_sampling_rate(inst, G, L)
-        cpu["samples_sample"] = "oracle sampling of 50 full samples"
+        cpu = cpu_baseline(inst, kron_out, G, L, pool)
+        pool[0].shutdown()
+        log(f"cpu baseline done at {time.perf_counter() - t_begin:.0f} s")
+    del G, L, s0
+    m144 = None
+    if not args.no_m144_probe:
+        if time.perf_counter() - t_begin < 400:
+            m144 = m144_probe()
+        else:
+            m144 = {"skipped": "the default run had used 400 s before this leg"}
+        log(f"M=144 probe done at {time.perf_counter() - t_begin:.0f} s")
 
-    sites_per_s = inst.M / t_build
     out = {
         "metric": "MPS sites/s at chi (construction), with samples/s",
-        "value": sites_per_s,
+        "value": inst.M / t_build,
         "unit": "sites/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -724,7 +848,8 @@ def main():
         "dtype": "c128 (fp64)",
         "data": "synthetic: seeded Haar unitary, closed-form pure/classical split",
         "config": {"workload": args.workload, "M": inst.M, "d": inst.d, "chi": inst.chi, "dd": inst.dd,
-                   "n_samples": n_samples, "precision_form": "fp64"},
+                   "samples_per_step": n_samples, "precision_form": "fp64",
+                   "construction": "pruned plain method (kept labels of both cuts)"},
         "samples_per_s": n_samples / t_samp,
         "construction_s": t_build,
         "sampling_s": t_samp,
@@ -733,10 +858,85 @@ def main():
         "kernels": kernels,
         "sampling_forms": forms,
         "learned_form": learned_form,
+        "m144_probe": m144,
         "tuning": {str(k): v for k, v in tuning.items()},
         "cpu_baseline": cpu,
+        "bench_wall_s": time.perf_counter() - t_begin,
     }
     print(json.dumps(out))
+
+
+def sampling_mid(samples, probs):
+    from bosonsupremacy_amd import batched
+    return batched.mid_uniforms(samples, probs)
+
+
+def learned_leg(inst, sqrtW, n_samples, normals, ref, forms, t_build, t_samp):
+    """The learned-cutoff form (per-site cutoffs and per-cut ranks predicted by the MLP, each
+    judged by the exact discarded-weight bound) with the fastest reduced-precision contraction
+    whose marginals stayed within a tenth of the bar, against the pruned plain FP64 form.  The
+    network's own saving is the hafnian and contraction work of its plan below the pruned plan's
+    (the pruned form already keeps every label of positive Schmidt weight and nothing else)."""
+    import torch
+    from bosonsupremacy_amd import kron, learned, metrics, mps, sampling
+    t0 = time.perf_counter()
+    model = learned.load_or_train()
+    t_train = time.perf_counter() - t0
+    log(f"learned form: network ready in {t_train:.1f} s")
+
+    def construction():
+        t0 = time.perf_counter()
+        ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+        sd, sc, rep = learned.plan_sites(model, inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
+        G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=sd, site_chi=sc)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, G2, L2, rep, ko, sd, sc
+
+    construction()
+    t_lc, G2, L2, rep, ko, sd, sc = construction()
+    kept = [int((np.asarray(r) > 1e-10).sum()) for r in ko[0]]
+    w_plain = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_chi=kept)
+    w_learn = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_d=sd, site_chi=sc)
+    accurate = [p for p in ("fp32", "bf16x6", "fp16x3")
+                if p in forms and forms[p]["first_order_marginal_max_rel_err_vs_fp64"] <= 0.1 * MARGINAL_BAR
+                and forms[p]["second_order_marginal_max_rel_err_vs_fp64"] <= MARGINAL_BAR]
+    lprec = max(accurate, key=lambda p: forms[p]["samples_per_s"]) if accurate else "fp32"
+    s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision=lprec)
+    s2.run(normals, ref["mid"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s2.run(normals, ref["mid"])
+    torch.cuda.synchronize()
+    t_ls = time.perf_counter() - t0
+    smp, probs = s2.run(normals, ref["mid"], return_probs=True)
+    e1 = metrics.rb_first_order(probs)
+    e2 = metrics.rb_second_order(smp, probs)
+    big = np.abs(ref["e2"]) > 1e-3 * np.abs(ref["e2"]).max()
+    out = dict(rep)
+    plain_gemm = sum(kept[i - 1] * kept[i] * inst.d for i in range(1, inst.M - 1)) + kept[-1] * inst.d
+    learn_gemm = sum(sc[i - 1] * sc[i] * sd[i] for i in range(1, inst.M - 1)) + sc[-1] * sd[-1]
+    out.update({
+        "precision": f"{lprec} contraction",
+        "training_s": t_train,
+        "construction_s": t_lc,
+        "construction_speedup_vs_pruned_plain": t_build / t_lc,
+        "network_saving": {
+            "hafnian_flops_pruned_plain": float(sum(w[1] for w in w_plain)),
+            "hafnian_flops_learned": float(sum(w[1] for w in w_learn)),
+            "hafnian_work_removed_frac": 1 - sum(w[1] for w in w_learn) / max(1.0, sum(w[1] for w in w_plain)),
+            "contraction_work_removed_frac": 1 - learn_gemm / plain_gemm,
+            "cuts_with_rank_below_kept": int(sum(1 for a, b in zip(sc, kept) if a < b)),
+            "sites_with_cutoff_below_d": int(sum(1 for x in sd if x < inst.d))},
+        "samples_per_s": n_samples / t_ls,
+        "sampling_speedup_vs_plain_fp64": t_samp / t_ls,
+        "end_to_end_speedup_vs_plain_fp64": (t_build + t_samp) / (t_lc + t_ls),
+        "first_order_marginal_max_rel_err_vs_plain_fp64": float(np.max(np.abs(e1 - ref["e1"]) / np.abs(ref["e1"]))),
+        "second_order_marginal_max_rel_err_vs_plain_fp64": float(np.max(np.abs(e2[big] - ref["e2"][big])
+                                                                        / np.abs(ref["e2"][big]))),
+        "off_fp64_path": int((smp != ref["samples"]).any(dim=1).sum()),
+    })
+    del G2, L2, s2, probs, smp
+    return out
 
 
 if __name__ == "__main__":
