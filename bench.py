@@ -600,6 +600,12 @@ def main():
                     help="only time batched hafnians of this size on synthetic targets (the launch shape "
                          "the autotuner picks; for profiling the hafnian kernels alone)")
     ap.add_argument("--hafnian-count", type=int, default=0, help="hafnians per call with --hafnian-size")
+    ap.add_argument("--chain-sites", default=None,
+                    help="FIRST-LAST: construct-and-sample sites FIRST..LAST of --workload site-major (chain.Chain), "
+                         "FP64 drawing and FP16x3 replaying its draws; resumes from --chain-resume")
+    ap.add_argument("--chain-samples", type=int, default=200)
+    ap.add_argument("--chain-resume", default=None, help="checkpoint written by the previous part of the chain")
+    ap.add_argument("--chain-save", default=None, help="where to write this part's checkpoint")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -614,6 +620,9 @@ def main():
         return
     if args.hafnian_size:
         hafnian_only(args)
+        return
+    if args.chain_sites:
+        chain_run(args)
         return
     if args.no_forms:
         args.no_learned = True
@@ -656,6 +665,61 @@ def form_flops(s, n):
     return sum(n * s.rows[i] * s.chis[i] * s.ds[i] for i in range(1, s.M))
 
 
+def chain_run(args):
+    """Part of a site-major construct-and-sample run (bosonsupremacy_amd.chain): the spectra of
+    every cut, the learned-cutoff plan (its ranks and cutoffs are walked as a third form when they
+    differ from the pruned plain form's), then sites FIRST..LAST built at full size one at a time,
+    every sample advanced through each before the next is built.  Progress per site on stderr;
+    one JSON line per part with per-site seconds and the marginal errors so far."""
+    import torch
+    from bosonsupremacy_amd import autotune, chain, instances, kron, learned, runtime
+    first, last = (int(x) for x in args.chain_sites.split("-"))
+    inst = instances.workload(args.workload)
+    runtime.context()
+    autotune.load_or_tune()
+    t0 = time.perf_counter()
+    ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+    torch.cuda.synchronize()
+    t_kron = time.perf_counter() - t0
+    log(f"spectra of {inst.M - 1} cuts: {t_kron:.2f} s")
+    kept = [int((np.asarray(r) > 1e-10).sum()) for r in ko[0]]
+    sd, sc, plan = learned.plan_sites(learned.load_or_train(), inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
+    forms = [chain.Form("fp64"), chain.Form("fp16x3", "fp16x3")]
+    same_plan = list(sd) == [inst.d] * inst.M and list(sc) == kept
+    if not same_plan:
+        forms.append(chain.Form("learned_fp16x3", "fp16x3", site_chi=sc, site_d=sd))
+    log(f"learned plan {'equals' if same_plan else 'differs from'} the pruned plain form "
+        f"(mean cutoff {plan['mean_cutoff']:.2f}, mean rank {plan['mean_rank']:.0f})")
+    c = chain.Chain(inst.sq_cov, inst.cov, inst.d, inst.chi, inst.dd, args.chain_samples, seed=11, forms=forms,
+                    kron=ko, chunk=args.chain_samples)
+    if args.chain_resume:
+        c.restore(args.chain_resume)
+        log(f"resumed at site {c.next_site}")
+    work = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_chi=kept)
+    runs = {k: int(np.ceil(work[k][1] / PARTIAL_RUN_FLOPS)) for k in range(first, last + 1)
+            if work[k][1] > PARTIAL_RUN_FLOPS}
+    t_part = time.perf_counter()
+
+    def on_run(site, i, n):
+        log(f"site {site}: run {i + 1}/{n} of its right labels enqueued")
+
+    def on_site(k, sec):
+        log(f"site {k}: {sec:.2f} s ({work[k][0]:.3e} hafnians, largest {work[k][2]}), "
+            f"part at {time.perf_counter() - t_part:.0f} s")
+    c.run(sites=range(first, last + 1), on_site=on_site, right_runs=runs, on_run=on_run)
+    if args.chain_save:
+        c.save(args.chain_save)
+    out = {"workload": args.workload, "chain_part": [first, last], "samples": args.chain_samples,
+           "forms": [f.name for f in forms], "learned_plan_equals_pruned": same_plan, "spectra_s": t_kron,
+           "site_seconds": {int(k): float(c.site_seconds[k]) for k in range(first, last + 1)},
+           "site_work": {int(k): {"hafnians": int(work[k][0]), "executed_flops": float(work[k][1]),
+                                  "largest_size": int(work[k][2])} for k in range(first, last + 1)},
+           "part_seconds": time.perf_counter() - t_part, "next_site": c.next_site, "errors": c.errors()}
+    if c.next_site == inst.M:
+        out["first_order_rb_fp64"] = c.first_order(0).tolist()
+    print(json.dumps(out))
+
+
 def standard_run(args):
     import torch
     from bosonsupremacy_amd import autotune, instances, kron, metrics, mps, runtime, sampling
@@ -665,7 +729,9 @@ def standard_run(args):
     inst = instances.workload(args.workload)
     n_samples = args.samples or SAMPLES_PER_STEP.get(args.workload, inst.n_samples)
     runtime.context()
+    log(f"workload {args.workload}: M={inst.M} d={inst.d} chi={inst.chi}, {n_samples} samples per step")
     tuning = autotune.load_or_tune()
+    log("launch tuning ready")
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     timer = KernelTimer()
     state = {}

@@ -40,11 +40,18 @@ def pool():
     serially."""
     global _pool
     if _pool is None and n_workers() > 1:
+        p = None
         try:
             p = ProcessPoolExecutor(max_workers=n_workers(), mp_context=multiprocessing.get_context("fork"))
+            # a worker that never answers (a fork taken while another thread held a lock) must
+            # not block the parent: give up on the pool and run serially
             for f in [p.submit(_noop) for _ in range(n_workers())]:
-                f.result()
+                f.result(timeout=60)
         except Exception:
+            if p is not None:
+                for proc in list(getattr(p, "_processes", {}).values()):
+                    proc.kill()
+                p.shutdown(wait=False, cancel_futures=True)
             return None
         _pool = p
         atexit.register(shutdown)
