@@ -684,12 +684,15 @@ def chain_run(args):
     log(f"spectra of {inst.M - 1} cuts: {t_kron:.2f} s")
     kept = [int((np.asarray(r) > 1e-10).sum()) for r in ko[0]]
     sd, sc, plan = learned.plan_sites(learned.load_or_train(), inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
-    forms = [chain.Form("fp64"), chain.Form("fp16x3", "fp16x3")]
     same_plan = list(sd) == [inst.d] * inst.M and list(sc) == kept
-    if not same_plan:
-        forms.append(chain.Form("learned_fp16x3", "fp16x3", site_chi=sc, site_d=sd))
     log(f"learned plan {'equals' if same_plan else 'differs from'} the pruned plain form "
         f"(mean cutoff {plan['mean_cutoff']:.2f}, mean rank {plan['mean_rank']:.0f})")
+    if args.chain_resume:   # every part of a chain walks the forms its first part chose
+        forms = chain.Chain.forms_of(args.chain_resume)
+    else:
+        forms = [chain.Form("fp64"), chain.Form("fp16x3", "fp16x3")]
+        if not same_plan and not args.no_learned:
+            forms.append(chain.Form("learned_fp16x3", "fp16x3", site_chi=sc, site_d=sd))
     c = chain.Chain(inst.sq_cov, inst.cov, inst.d, inst.chi, inst.dd, args.chain_samples, seed=11, forms=forms,
                     kron=ko, chunk=args.chain_samples)
     if args.chain_resume:
@@ -710,7 +713,10 @@ def chain_run(args):
     if args.chain_save:
         c.save(args.chain_save)
     out = {"workload": args.workload, "chain_part": [first, last], "samples": args.chain_samples,
-           "forms": [f.name for f in forms], "learned_plan_equals_pruned": same_plan, "spectra_s": t_kron,
+           "forms": [f.name for f in forms], "learned_plan_equals_pruned": same_plan,
+           "learned_plan": {"mean_rank": plan["mean_rank"], "mean_cutoff": plan["mean_cutoff"],
+                            "cuts_with_rank_below_kept": int(sum(a < b for a, b in zip(sc, kept)))},
+           "spectra_s": t_kron,
            "site_seconds": {int(k): float(c.site_seconds[k]) for k in range(first, last + 1)},
            "site_work": {int(k): {"hafnians": int(work[k][0]), "executed_flops": float(work[k][1]),
                                   "largest_size": int(work[k][2])} for k in range(first, last + 1)},
@@ -942,66 +948,81 @@ def learned_leg(inst, sqrtW, n_samples, normals, ref, forms, t_build, t_samp):
     judged by the exact discarded-weight bound) with the fastest reduced-precision contraction
     whose marginals stayed within a tenth of the bar, against the pruned plain FP64 form.  The
     network's own saving is the hafnian and contraction work of its plan below the pruned plan's
-    (the pruned form already keeps every label of positive Schmidt weight and nothing else)."""
+    (the pruned form already keeps every label of Schmidt weight above 1e-10 and nothing else).
+
+    It is run twice: at the default tolerances (1e-10 of Schmidt weight per cut, the most a
+    truncation may drop when the marginals must stay within 1e-6, since a truncation moves the
+    sampled conditionals by the square root of the weight it drops) and at the tolerance the
+    network is trained for (1e-7), where it has work to remove and costs ~1e-5 in the marginals."""
     import torch
     from bosonsupremacy_amd import kron, learned, metrics, mps, sampling
     t0 = time.perf_counter()
     model = learned.load_or_train()
     t_train = time.perf_counter() - t0
     log(f"learned form: network ready in {t_train:.1f} s")
-
-    def construction():
-        t0 = time.perf_counter()
-        ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
-        sd, sc, rep = learned.plan_sites(model, inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi)
-        G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=sd, site_chi=sc)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0, G2, L2, rep, ko, sd, sc
-
-    construction()
-    t_lc, G2, L2, rep, ko, sd, sc = construction()
-    kept = [int((np.asarray(r) > 1e-10).sum()) for r in ko[0]]
-    w_plain = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_chi=kept)
-    w_learn = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_d=sd, site_chi=sc)
     accurate = [p for p in ("fp32", "bf16x6", "fp16x3")
                 if p in forms and forms[p]["first_order_marginal_max_rel_err_vs_fp64"] <= 0.1 * MARGINAL_BAR
                 and forms[p]["second_order_marginal_max_rel_err_vs_fp64"] <= MARGINAL_BAR]
     lprec = max(accurate, key=lambda p: forms[p]["samples_per_s"]) if accurate else "fp32"
-    s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision=lprec)
-    s2.run(normals, ref["mid"])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s2.run(normals, ref["mid"])
-    torch.cuda.synchronize()
-    t_ls = time.perf_counter() - t0
-    smp, probs = s2.run(normals, ref["mid"], return_probs=True)
-    e1 = metrics.rb_first_order(probs)
-    e2 = metrics.rb_second_order(smp, probs)
     big = np.abs(ref["e2"]) > 1e-3 * np.abs(ref["e2"]).max()
-    out = dict(rep)
-    plain_gemm = sum(kept[i - 1] * kept[i] * inst.d for i in range(1, inst.M - 1)) + kept[-1] * inst.d
-    learn_gemm = sum(sc[i - 1] * sc[i] * sd[i] for i in range(1, inst.M - 1)) + sc[-1] * sd[-1]
-    out.update({
-        "precision": f"{lprec} contraction",
-        "training_s": t_train,
-        "construction_s": t_lc,
-        "construction_speedup_vs_pruned_plain": t_build / t_lc,
-        "network_saving": {
-            "hafnian_flops_pruned_plain": float(sum(w[1] for w in w_plain)),
-            "hafnian_flops_learned": float(sum(w[1] for w in w_learn)),
-            "hafnian_work_removed_frac": 1 - sum(w[1] for w in w_learn) / max(1.0, sum(w[1] for w in w_plain)),
-            "contraction_work_removed_frac": 1 - learn_gemm / plain_gemm,
-            "cuts_with_rank_below_kept": int(sum(1 for a, b in zip(sc, kept) if a < b)),
-            "sites_with_cutoff_below_d": int(sum(1 for x in sd if x < inst.d))},
-        "samples_per_s": n_samples / t_ls,
-        "sampling_speedup_vs_plain_fp64": t_samp / t_ls,
-        "end_to_end_speedup_vs_plain_fp64": (t_build + t_samp) / (t_lc + t_ls),
-        "first_order_marginal_max_rel_err_vs_plain_fp64": float(np.max(np.abs(e1 - ref["e1"]) / np.abs(ref["e1"]))),
-        "second_order_marginal_max_rel_err_vs_plain_fp64": float(np.max(np.abs(e2[big] - ref["e2"][big])
-                                                                        / np.abs(ref["e2"][big]))),
-        "off_fp64_path": int((smp != ref["samples"]).any(dim=1).sum()),
-    })
-    del G2, L2, s2, probs, smp
+
+    def construction(tau):
+        t0 = time.perf_counter()
+        ko = kron.kron_all(inst.sq_cov, inst.d, inst.chi)
+        sd, sc, rep = learned.plan_sites(model, inst.sq_cov, inst.cov, ko[0], inst.d, inst.chi, tau_chi=tau)
+        G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=sd, site_chi=sc)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, G2, L2, rep, ko, sd, sc
+
+    def leg(tau, warm):
+        if warm:
+            construction(tau)
+        t_lc, G2, L2, rep, ko, sd, sc = construction(tau)
+        kept = [int((np.asarray(r) > 1e-10).sum()) for r in ko[0]]
+        w_plain = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_chi=kept)
+        w_learn = site_hafnian_work_by_site(ko[0], ko[1], inst.d, site_d=sd, site_chi=sc)
+        s2 = sampling.Sampler(G2, L2, sqrtW, inst.dd, precision=lprec)
+        s2.run(normals, ref["mid"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s2.run(normals, ref["mid"])
+        torch.cuda.synchronize()
+        t_ls = time.perf_counter() - t0
+        smp, probs = s2.run(normals, ref["mid"], return_probs=True)
+        e1 = metrics.rb_first_order(probs)
+        e2 = metrics.rb_second_order(smp, probs)
+        out = dict(rep)
+        plain_gemm = sum(kept[i - 1] * kept[i] * inst.d for i in range(1, inst.M - 1)) + kept[-1] * inst.d
+        learn_gemm = sum(sc[i - 1] * sc[i] * sd[i] for i in range(1, inst.M - 1)) + sc[-1] * sd[-1]
+        r1 = float(np.max(np.abs(e1 - ref["e1"]) / np.abs(ref["e1"])))
+        r2 = float(np.max(np.abs(e2[big] - ref["e2"][big]) / np.abs(ref["e2"][big])))
+        out.update({
+            "tau_schmidt_weight_per_cut": tau,
+            "construction_s": t_lc,
+            "construction_speedup_vs_pruned_plain": t_build / t_lc,
+            "network_saving": {
+                "hafnian_flops_pruned_plain": float(sum(w[1] for w in w_plain)),
+                "hafnian_flops_learned": float(sum(w[1] for w in w_learn)),
+                "hafnian_work_removed_frac": 1 - sum(w[1] for w in w_learn) / max(1.0, sum(w[1] for w in w_plain)),
+                "contraction_work_removed_frac": 1 - learn_gemm / plain_gemm,
+                "cuts_with_rank_below_kept": int(sum(1 for a, b in zip(sc, kept) if a < b)),
+                "sites_with_cutoff_below_d": int(sum(1 for x in sd if x < inst.d))},
+            "samples_per_s": n_samples / t_ls,
+            "sampling_speedup_vs_plain_fp64": t_samp / t_ls,
+            "end_to_end_speedup_vs_plain_fp64": (t_build + t_samp) / (t_lc + t_ls),
+            "first_order_marginal_max_rel_err_vs_plain_fp64": r1,
+            "second_order_marginal_max_rel_err_vs_plain_fp64": r2,
+            "within_bar": bool(max(r1, r2) <= MARGINAL_BAR),
+            "off_fp64_path": int((smp != ref["samples"]).any(dim=1).sum()),
+        })
+        log(f"learned form at tau {tau:g}: hafnian work removed "
+            f"{out['network_saving']['hafnian_work_removed_frac']:.3f}, marginals {r1:.2e} / {r2:.2e}")
+        del G2, L2, s2, probs, smp
+        return out
+
+    out = {"precision": f"{lprec} contraction", "training_s": t_train,
+           "at_bar": leg(learned.TAU_CHI, True),
+           "at_training_tolerance": leg(learned.TAU_CHI_TRAIN, False)}
     return out
 
 

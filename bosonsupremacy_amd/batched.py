@@ -22,20 +22,26 @@ import time
 import numpy as np
 import torch
 
-from . import mps, runtime, sampling
+from . import metrics, mps, runtime, sampling
 
 
 class GraphSampler:
     """A Sampler whose pass over all sites is captured as a HIP graph for a fixed batch size n.
 
     The graph is recorded on a side stream with its own library context (a context enqueues on
-    the stream current at its creation); static buffers hold the random inputs and outputs."""
+    the stream current at its creation); static buffers hold the random inputs and outputs, and
+    the graph keeps its own memory pool.  ``prepare_s`` times the device preparation of the
+    site tensors (the form's conversions and splits), ``capture_s`` the warm-up pass (kernels
+    launched outside capture) and the capture."""
 
     def __init__(self, gammas, lambdas, sqrtW, dd, n, precision="fp64", return_probs=False):
+        import time
         self.stream = torch.cuda.Stream()
         with torch.cuda.stream(self.stream):
             self.ctx = runtime.Context(runtime.context().device.index)
         self.ctx.copy_settings(runtime.context())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         self.sampler = sampling.Sampler(gammas, lambdas, sqrtW, dd, precision=precision, ctx=self.ctx)
         M = self.sampler.M
         dev = self.sampler.dev
@@ -44,6 +50,7 @@ class GraphSampler:
         self.uniforms = torch.zeros((M, n), dtype=torch.float64, device=dev)
         self.return_probs = return_probs
         torch.cuda.synchronize()
+        t1 = time.perf_counter()
         with torch.cuda.stream(self.stream):  # load the kernels outside capture (small batch)
             w = min(n, 256)
             self.sampler.run(self.normals[:, :w], self.uniforms[:, :w], return_probs=return_probs)
@@ -52,6 +59,7 @@ class GraphSampler:
         with torch.cuda.graph(self.graph, stream=self.stream):
             self.out = self.sampler.run(self.normals, self.uniforms, return_probs=return_probs)
         torch.cuda.synchronize()
+        self.prepare_s, self.capture_s = t1 - t0, time.perf_counter() - t1
 
     def run(self, normals, uniforms):
         """Replay on new inputs (device [2M, n] and [M, n]); returns the static outputs."""
@@ -94,10 +102,11 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
         tot.update({"construction_learned": 0.0, "setup_learned": 0.0, "sampling_learned": 0.0})
         errs["learned"] = []
     rows, skipped = [], []
+    cap = {f: [0.0, 0.0] for f in names + (("learned",) if learned_model is not None else ())}
     for i, inst in enumerate(instances):
         t0 = now()
         try:
-            G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
+            G, L, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, pruned=True)
         except BsmpsError as e:  # e.g. hafnians beyond the supported size
             torch.cuda.synchronize()
             skipped.append({"index": i, "M": inst.M, "d": inst.d, "chi": inst.chi, "reason": str(e)})
@@ -112,6 +121,8 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
             ta = now()
             graphs[f] = GraphSampler(G, L, sqrtW, inst.dd, n_samples, f, return_probs=True)
             times["setup_" + f] = now() - ta
+            cap[f][0] += graphs[f].prepare_s
+            cap[f][1] += graphs[f].capture_s
         normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=inst.seed)
         row = {"M": inst.M, "d": inst.d, "chi": inst.chi}
         ta = now()
@@ -119,6 +130,15 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
         times["sampling_fp64"] = now() - ta
         mid = mid_uniforms(s64, p64)
         m64 = p64.mean(dim=1)
+        e1_64, e2_64 = metrics.rb_first_order(p64), metrics.rb_second_order(s64, p64)
+        big = np.abs(e2_64) > 1e-3 * np.abs(e2_64).max()
+
+        def rel_errs(sf, pf):
+            """Largest relative differences of the Rao-Blackwellised first- and second-order
+            marginals from FP64's (both along the FP64 sample paths)."""
+            e1, e2 = metrics.rb_first_order(pf), metrics.rb_second_order(sf, pf)
+            return (float(np.max(np.abs(e1 - e1_64) / np.abs(e1_64))),
+                    float(np.max(np.abs(e2[big] - e2_64[big]) / np.abs(e2_64[big]))) if big.any() else 0.0)
         line = []
         for f in forms:
             ta = now()
@@ -128,9 +148,10 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
             sf, pf = graphs[f].run(normals, mid)                # along the FP64 sample paths
             off_path = int((sf != s64).any(dim=1).sum())
             err = float(torch.abs(pf.mean(dim=1) - m64).max())
-            errs[f].append(err)
-            row.update({f"{f}_marginal_abs_err": err, f"{f}_draws_differing_same_uniforms": flips,
-                        f"{f}_off_fp64_path": off_path})
+            r1, r2 = rel_errs(sf, pf)
+            errs[f].append((err, r1, r2))
+            row.update({f"{f}_marginal_abs_err": err, f"{f}_first_order_rel_err": r1, f"{f}_second_order_rel_err": r2,
+                        f"{f}_draws_differing_same_uniforms": flips, f"{f}_off_fp64_path": off_path})
             line.append(f"{f} {times['sampling_' + f]:.3f} s err {err:.1e} flips {flips}")
         if learned_model is not None:
             ta = now()
@@ -140,14 +161,18 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
             Gl, Ll, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=ko, site_d=site_d, site_chi=site_chi)
             tb = now()
             gl = GraphSampler(Gl, Ll, sqrtW, inst.dd, n_samples, learned_precision, return_probs=True)
+            cap["learned"][0] += gl.prepare_s
+            cap["learned"][1] += gl.capture_s
             tc = now()
             gl.run(normals, uniforms)
             td = now()
             times.update({"construction_learned": tb - ta, "setup_learned": tc - tb, "sampling_learned": td - tc})
             sl, pl = gl.run(normals, mid)
             err = float(torch.abs(pl.mean(dim=1) - m64).max())
-            errs["learned"].append(err)
-            row.update({"learned_marginal_abs_err": err, "learned_off_fp64_path": int((sl != s64).any(dim=1).sum()),
+            r1, r2 = rel_errs(sl, pl)
+            errs["learned"].append((err, r1, r2))
+            row.update({"learned_marginal_abs_err": err, "learned_first_order_rel_err": r1,
+                        "learned_second_order_rel_err": r2, "learned_off_fp64_path": int((sl != s64).any(dim=1).sum()),
                         "learned_mean_cutoff": plan["mean_cutoff"], "learned_mean_rank": plan["mean_rank"]})
             line.append(f"learned+{learned_precision} construction {tb - ta:.2f} s err {err:.1e}")
             del Gl, Ll, gl
@@ -168,14 +193,20 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
     if learned_model is not None:
         out[f"instances_per_s_learned_{learned_precision}"] = done / (
             tot["construction_learned"] + tot["setup_learned"] + tot["sampling_learned"])
-        out["learned_marginal_abs_err_max"] = float(np.max(errs["learned"]))
-        out["learned_marginal_abs_err_median"] = float(np.median(errs["learned"]))
+        out["learned_marginal_abs_err_max"] = float(np.max([e[0] for e in errs["learned"]]))
+        out["learned_marginal_abs_err_median"] = float(np.median([e[0] for e in errs["learned"]]))
+        out["learned_first_order_rel_err_max"] = float(np.max([e[1] for e in errs["learned"]]))
+        out["learned_second_order_rel_err_max"] = float(np.max([e[2] for e in errs["learned"]]))
         out["learned_off_fp64_path_total"] = int(sum(r["learned_off_fp64_path"] for r in rows))
     for f in forms:
-        out[f"{f}_marginal_abs_err_max"] = float(np.max(errs[f]))
-        out[f"{f}_marginal_abs_err_median"] = float(np.median(errs[f]))
+        out[f"{f}_marginal_abs_err_max"] = float(np.max([e[0] for e in errs[f]]))
+        out[f"{f}_marginal_abs_err_median"] = float(np.median([e[0] for e in errs[f]]))
+        out[f"{f}_first_order_rel_err_max"] = float(np.max([e[1] for e in errs[f]]))
+        out[f"{f}_second_order_rel_err_max"] = float(np.max([e[2] for e in errs[f]]))
         out[f"{f}_off_fp64_path_total"] = int(sum(r[f"{f}_off_fp64_path"] for r in rows))
         out[f"{f}_draws_differing_same_uniforms_total"] = int(
             sum(r[f"{f}_draws_differing_same_uniforms"] for r in rows))
+    out["graph_prepare_s"] = {f: v[0] for f, v in cap.items()}
+    out["graph_capture_s"] = {f: v[1] for f, v in cap.items()}
     out["per_instance"] = rows
     return out

@@ -158,6 +158,11 @@ class Chain:
               "mean_probs": self.mean_probs, "cond_mean": self.cond_mean, "off_path": self.off_path,
               "site_seconds": self.site_seconds,
               "forms": np.array([f"{f.name}:{f.precision}" for f in self.forms])}
+        for fi, f in enumerate(self.forms):   # a learned plan travels with the checkpoint
+            if f.site_chi is not None:
+                st[f"plan{fi}_site_chi"] = np.asarray(f.site_chi)
+            if f.site_d is not None:
+                st[f"plan{fi}_site_d"] = np.asarray(f.site_d)
         for fi, w in enumerate(self.walkers):
             for key, v in w.state().items():
                 st[f"w{fi}_{key}"] = v
@@ -165,6 +170,19 @@ class Chain:
 
     def save(self, path):
         np.savez(path, **self.state())
+
+    @staticmethod
+    def forms_of(path):
+        """The forms a checkpoint was walked with (learned plans included), so every part of a
+        chain walks the same forms."""
+        st = np.load(path, allow_pickle=False)
+        out = []
+        for fi, spec in enumerate(st["forms"]):
+            name, precision = str(spec).split(":")
+            out.append(Form(name, precision,
+                            site_chi=st[f"plan{fi}_site_chi"] if f"plan{fi}_site_chi" in st.files else None,
+                            site_d=st[f"plan{fi}_site_d"] if f"plan{fi}_site_d" in st.files else None))
+        return out
 
     def restore(self, path):
         st = np.load(path, allow_pickle=False)

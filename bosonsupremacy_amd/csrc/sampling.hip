@@ -538,6 +538,9 @@ __global__ __launch_bounds__(256) void site_gemm_bf16x3_kernel(int64_t n, int ch
 // imaginary accumulator.
 // ---------------------------------------------------------------------------------------
 constexpr int X6_TM = 128, X6_TN = 64, X6_KC = 32;
+// FP16x3 contractions with more than kSplitKMin values of k run in k segments of kSplitKSeg
+// (see site_gemm_split_kernel)
+constexpr int64_t kSplitKMin = 4096, kSplitKSeg = 2048;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
@@ -662,16 +665,33 @@ inline int64_t gemm_tile_grid(int64_t gx, int64_t gy) {
     return kGemmXcds * ((gx + kGemmXcds - 1) / kGemmXcds) * gy;
 }
 
+// Sum of the k-segment partials of the split-form contraction: T[i] = (float) sum_s P_s[i], the
+// partials added in FP64 in segment order (the same bits however the grid is shaped).
+__global__ __launch_bounds__(256) void split_k_combine_kernel(const float* __restrict__ P, int64_t count, int segs,
+                                                              float* __restrict__ T) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) {
+        double acc = 0.0;
+        for (int s = 0; s < segs; ++s) acc += (double)P[(int64_t)s * count + i];
+        T[i] = (float)acc;
+    }
+}
+
 // One kernel for both split formats: F::NP parts per value, 2 F::NP planes per operand (parts of
 // the real part, then of the imaginary part).  With F::SCALED the planes hold values divided by a
 // per-row (A) and per-column (B) scale, which the epilogue multiplies back.
+//
+// The k range may be cut into segments (blockIdx.y = segment, kseg values of k each, a multiple of
+// X6_KC): segment s accumulates k in [s kseg, min((s + 1) kseg, kpad)) and writes its partial T to
+// T + s * (n * ncols * 2) floats; split_k_combine_kernel then adds the partials in FP64.  A shorter
+// FP32 accumulation chain per output is the point (at k = 10^4 the ~2000 FP32 roundings of one
+// chain, not the products, set the error).  kseg = kpad is the unsegmented product.
 template <class F>
 __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
                                                                  const typename F::E* __restrict__ apl,
                                                                  const float* __restrict__ rowscale,
                                                                  const typename F::E* __restrict__ bpl,
                                                                  const float* __restrict__ colscale,
-                                                                 float* __restrict__ T) {
+                                                                 float* __restrict__ T, int kseg) {
     typedef typename F::E E;
     typedef typename F::V V;
     constexpr int NP = F::NP, NPL = 2 * F::NP;
@@ -685,6 +705,9 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
     const int col0 = cb * X6_TN;  // complex column
     const int wm = (wave / 2) * 64, wn = (wave % 2) * 32;
     const int64_t ksteps = kpad / X6_KC;
+    const int kbeg = (int)blockIdx.y * kseg;
+    const int kend = kbeg + kseg < kpad ? kbeg + kseg : kpad;
+    T += (int64_t)blockIdx.y * n * ncols * 2;
 
     float4_t acc_re[4][2], acc_im[4][2];
 #pragma unroll
@@ -738,11 +761,11 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
     };
 
     const int g = lane / 16;
-    fetch(0);
-    for (int k0 = 0; k0 < kpad; k0 += X6_KC) {
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += X6_KC) {
         stash(k0);
         __syncthreads();
-        if (k0 + X6_KC < kpad) fetch(k0 + X6_KC);
+        if (k0 + X6_KC < kend) fetch(k0 + X6_KC);
         // fragments: lane holds k 8 (lane / 16) .. + 8 of row / column lane % 16
         V bfr[2][NPL];
 #pragma unroll
@@ -1637,7 +1660,7 @@ extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_pre
     hipLaunchKernelGGL(site_gemm_split_kernel<Bf16x6Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0,
                        ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
                        reinterpret_cast<const __bf16*>(pre_planes), nullptr,
-                       reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T);
+                       reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T, (int)kpad);
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
@@ -1712,10 +1735,25 @@ extern "C" int bsmps_site_contract_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_pre
         bsmps_set_error("bsmps_site_contract_fp16x3: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_split_kernel<Fp16x3Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0,
-                       ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
+    // long k: segments of kSplitKSeg values, partials in the context workspace, FP64 combine
+    const int segs = kpad > kSplitKMin ? (int)((kpad + kSplitKSeg - 1) / kSplitKSeg) : 1;
+    float* out = T;
+    const int64_t count = n * ncols * 2;
+    if (segs > 1) {
+        void* ws = nullptr;
+        BSMPS_TRY(bsmps_ctx_workspace(ctx, sizeof(float) * (size_t)count * segs, &ws));
+        out = static_cast<float*>(ws);
+    }
+    hipLaunchKernelGGL(site_gemm_split_kernel<Fp16x3Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs),
+                       dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
                        reinterpret_cast<const _Float16*>(pre_planes), rowscale,
-                       reinterpret_cast<const _Float16*>(gamma_planes), colscale, T);
+                       reinterpret_cast<const _Float16*>(gamma_planes), colscale, out,
+                       segs > 1 ? (int)kSplitKSeg : (int)kpad);
     BSMPS_HIP(hipGetLastError());
+    if (segs > 1) {
+        hipLaunchKernelGGL(split_k_combine_kernel, dim3((unsigned)std::min<int64_t>((count + 255) / 256, 65536)),
+                           dim3(256), 0, ctx->stream, out, count, segs, T);
+        BSMPS_HIP(hipGetLastError());
+    }
     return BSMPS_OK;
 }

@@ -33,19 +33,25 @@ from . import gaussian, runtime
 N_FEATURES = 16
 WIDTH = 128
 N_OUT = 2
-# Tolerances of the exact checks.  A dropped Schmidt vector changes the conditional
-# distributions linearly in its weight (bond indices are summed incoherently in p = sum_a |.|^2),
-# but a dropped local component j >= d_k is not orthogonal to the kept ones after the
-# displacement D(alpha) mixes photon numbers, so dropping photon weight eps moves the displaced
-# probabilities by O(sqrt(eps)) through interference (measured on the M=64 workload: 1e-7 of
-# dropped photon weight moved first-order marginals by 4e-5).  A 1e-6 marginal target
-# therefore allows ~1e-12 of photon weight but ~1e-7 of Schmidt weight.
-TAU_D = 1e-12    # extra photon weight a site may drop by its cutoff
-TAU_CHI = 1e-7   # extra Schmidt weight a cut may drop by its rank
-D_SCALE = 10.0   # target scaling: cutoff / 10, log2(rank fraction) / 10
-R_SCALE = 10.0
-MARGIN_D = 0.5   # safety margins added to the predictions before rounding up
-MARGIN_R = 0.5
+# Tolerances of the exact checks.  Both kinds of truncation drop an amplitude, not a
+# probability: the conditional distributions the sampler draws from involve modes on both sides
+# of every earlier cut (the draws on the left, site k on the right), so a dropped Schmidt vector
+# of weight delta moves them by O(sqrt(delta)) through its cross terms with the kept ones, and a
+# dropped local component j >= d_k, mixed with the kept ones by the displacement D(alpha), does
+# the same.  Measured along common sample paths: 1e-7 of dropped photon weight moved M=64
+# first-order marginals by 4e-5, and ranks cut to 1e-7 of Schmidt weight per cut (23% less
+# hafnian work at M=32) moved them by 1.3e-5.  A 1e-6 marginal bar therefore allows ~1e-12 of
+# photon weight and ~1e-9 of Schmidt weight in all -- less than one kept label (every kept
+# label weighs more than 1e-10) -- so TAU_CHI keeps every kept vector.  The network is trained
+# to predict the ranks of the looser tolerance TAU_CHI_TRAIN; plan_sites(tau_chi=TAU_CHI_TRAIN)
+# uses them, at a marginal error of ~1e-5.
+TAU_D = 1e-12          # extra photon weight a site may drop by its cutoff
+TAU_CHI = 1e-10        # extra Schmidt weight a cut may drop by its rank at the 1e-6 bar
+TAU_CHI_TRAIN = 1e-7   # the tolerance the rank targets are computed at
+D_SCALE = 10.0   # target scaling: cutoff / 10, log2(needed rank / kept rank) / 2
+R_SCALE = 2.0
+MARGIN_D = 0.5   # safety margins added to the predictions before rounding up (photons; log2 of rank)
+MARGIN_R = 0.05
 
 
 # ---------------------------------------------------------------------------------------
@@ -65,11 +71,20 @@ def site_features(sq_cov, cov, kres):
         f += [W[k, k], W[k + M, k + M]]                                 # 2: displacement variance
         f += [(sq_cov[k, k] + sq_cov[k + M, k + M] - 2) / 4,            # 2: mean photons,
               (cov[k, k] + cov[k + M, k + M] - 2) / 4]                  #    pure and full state
-        f += [k / max(M - 1, 1)]                                        # 1: position
         lead = float(np.max(kres[k])) if k < M - 1 and len(kres[k]) else 1.0
         f += [np.log10(max(1.0 - lead, 1e-16))]                         # 1: entanglement of cut k
+        f += [_tail_feature(kres[k]) if k < M - 1 else 0.0]             # 1: weight near the kept end
         out[k] = f
     return out
+
+
+def _tail_feature(res, err_tol=1e-10):
+    """log10 of the Schmidt weight in the last tenth of the kept spectrum (how fast it ends)."""
+    r = np.asarray(res, dtype=np.float64)
+    r = r[r > err_tol]
+    if len(r) == 0:
+        return -16.0
+    return float(np.log10(max(np.sum(r[int(0.9 * len(r)):]), 1e-16)))
 
 
 def mode_covariance(sq_cov, k):
@@ -100,13 +115,15 @@ def needed_rank(res, tau=TAU_CHI, err_tol=1e-10):
 
 
 def site_labels(sq_cov, kres, chi, d_cap=12):
-    """Targets [M, 2] float32: needed cutoff / D_SCALE, log2(needed rank / chi) / R_SCALE."""
+    """Targets [M, 2] float32: needed cutoff / D_SCALE, log2(needed rank / kept rank) / R_SCALE
+    (the kept rank -- Schmidt weights above 1e-10, at most chi -- is what the pruned plain form
+    keeps; the needed rank drops at most TAU_CHI_TRAIN more)."""
     M = len(sq_cov) // 2
     out = np.zeros((M, N_OUT), dtype=np.float32)
     for k in range(M):
         out[k, 0] = needed_cutoff(sq_cov, k, d_cap) / D_SCALE
         if k < M - 1:
-            out[k, 1] = np.log2(max(needed_rank(kres[k]), 1) / chi) / R_SCALE
+            out[k, 1] = np.log2(max(needed_rank(kres[k], TAU_CHI_TRAIN), 1) / max(kept_rank(kres[k]), 1)) / R_SCALE
     return out
 
 
@@ -224,7 +241,7 @@ class MLP:
         self._cast()
 
 
-MODEL_VERSION = 2  # bump when features, labels or tolerances change
+MODEL_VERSION = 3  # bump when features, labels or tolerances change
 
 
 def _cache_path():
@@ -290,7 +307,7 @@ def plan_sites(model, sq_cov, cov, kres, d, chi, tau_d=TAU_D, tau_chi=TAU_CHI):
         extra_d.append(drop)
     for k in range(M - 1):
         kept = kept_rank(kres[k])
-        rk = int(np.clip(np.ceil(chi * 2.0 ** (Y[k, 1] * R_SCALE + MARGIN_R)), 1, kept))
+        rk = int(np.clip(np.ceil(kept * 2.0 ** (Y[k, 1] * R_SCALE + MARGIN_R)), 1, kept))
         r = np.asarray(kres[k], dtype=np.float64)
         drop = float(np.sum(r[rk:kept]))               # exact Schmidt weight dropped
         if drop > tau_chi:

@@ -90,38 +90,58 @@ def test_training_fits_a_smooth_function():
 
 
 def test_learned_form_gain_on_unseen_instance():
-    """Train on plain-method runs of other seeds, plan the M=32 workload (seed 0) and compare
-    with the plain method: every kept prediction passes the exact check, the per-site
-    conditional distributions move by at most the bounded weight, and the work shrinks."""
+    """Train on plain-method runs of other seeds, plan the M=32 workload (seed 0), and compare with
+    the pruned plain method -- the plain method without the work a zero Lambda multiplies away,
+    which already keeps exactly the labels of Schmidt weight above 1e-10.
+
+    * At the 1e-6 bar (the default tolerances) every kept prediction passes the exact check and
+      the first-order marginals move by less than 1e-6; the network's own saving is reported (the
+      bar leaves it almost nothing to drop: a truncation moves the sampled conditionals by the
+      square root of the weight it drops).
+    * At the looser tolerance the network is trained for (1e-7 of Schmidt weight per cut) its
+      saving and the marginal change it costs are reported."""
     from bosonsupremacy_amd import instances, learned, mps, runtime, sampling
-    from bench import site_hafnian_work
+    from bench import site_hafnian_work_by_site
     model = learned.MLP(seed=0)
     X, T = learned.training_pairs(learned.training_instances(seeds=range(1, 5)))
     hist = model.fit(X, T, steps=2000, log_every=500)
     print("training loss", hist, flush=True)
     inst = instances.workload("m32_d6_chi512")
-    G, L, kron_out = mps.build_mps(inst.sq_cov, inst.d, inst.chi)
-    kres = kron_out[0]
-    site_d, site_chi, report = learned.plan_sites(model, inst.sq_cov, inst.cov, kres, inst.d, inst.chi)
-    print("plan", site_d, site_chi, report, flush=True)
-    assert report["max_extra_photon_weight"] <= learned.TAU_D
-    assert report["max_extra_schmidt_weight"] <= learned.TAU_CHI
-    G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=kron_out, site_d=site_d, site_chi=site_chi)
-    # work: hafnian flops and sampling-contraction flops, plain vs learned
-    _, f_plain = site_hafnian_work(kres, kron_out[1], inst.d)
-    chis = [learned.kept_rank(r) for r in kres]
-    gemm_plain = sum(inst.chi * inst.chi * inst.d for _ in range(1, inst.M))
-    gemm_learned = sum(site_chi[i - 1] * (site_chi[i] if i < inst.M - 1 else 1) * site_d[i] for i in range(1, inst.M))
-    print(f"gemm work ratio {gemm_plain / gemm_learned:.3f}, kept ranks {chis}", flush=True)
-    assert gemm_learned < gemm_plain
-    # same random numbers through both forms: Rao-Blackwellised first-order marginals agree
+    Gp, Lp, kron_out = mps.build_mps(inst.sq_cov, inst.d, inst.chi, pruned=True)
+    kres, knum = kron_out[0], kron_out[1]
+    kept = [learned.kept_rank(r) for r in kres]
+    w_pruned = sum(w[1] for w in site_hafnian_work_by_site(kres, knum, inst.d, site_chi=kept))
+    c_pruned = sum(kept[i - 1] * kept[i] * inst.d for i in range(1, inst.M - 1)) + kept[-1] * inst.d
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     normals, uniforms = sampling.random_inputs(inst.M, 20000, seed=9)
-    _, pa = sampling.Sampler(G, L, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
-    _, pb = sampling.Sampler(G2, L2, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
-    pa, pb = runtime.to_host(pa), runtime.to_host(pb)
-    ma = (pa * np.arange(inst.dd)).sum(axis=2).mean(axis=1)
-    mb = (pb * np.arange(inst.dd)).sum(axis=2).mean(axis=1)
-    rel = np.max(np.abs(ma - mb) / ma)
-    print("first-order marginal max rel diff learned vs plain", rel, flush=True)
-    assert rel < 1e-6
+    _, pa = sampling.Sampler(Gp, Lp, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
+    ma = (runtime.to_host(pa) * np.arange(inst.dd)).sum(axis=2).mean(axis=1)
+    del pa
+    results = {}
+    for tau in (learned.TAU_CHI, learned.TAU_CHI_TRAIN):
+        site_d, site_chi, report = learned.plan_sites(model, inst.sq_cov, inst.cov, kres, inst.d, inst.chi,
+                                                      tau_chi=tau)
+        print(f"tau {tau:g}: plan {site_chi} {report}", flush=True)
+        assert report["max_extra_photon_weight"] <= learned.TAU_D
+        assert report["max_extra_schmidt_weight"] <= tau
+        assert all(1 <= r <= k for r, k in zip(site_chi, kept)) and all(1 <= x <= inst.d for x in site_d)
+        G2, L2, _ = mps.build_mps(inst.sq_cov, inst.d, inst.chi, kron=kron_out, site_d=site_d, site_chi=site_chi)
+        # the learned tensors are leading blocks of the pruned tensors (same entries)
+        for k in (1, inst.M // 2, inst.M - 2):
+            g2 = runtime.to_host(G2[k])
+            assert np.array_equal(g2, runtime.to_host(Gp[k])[:g2.shape[0], :g2.shape[1], :g2.shape[2]])
+        w_learned = sum(w[1] for w in site_hafnian_work_by_site(kres, knum, inst.d, site_d=site_d, site_chi=site_chi))
+        c_learned = sum(site_chi[i - 1] * site_chi[i] * site_d[i] for i in range(1, inst.M - 1)) + \
+            site_chi[-1] * site_d[-1]
+        assert w_learned <= w_pruned and c_learned <= c_pruned
+        _, pb = sampling.Sampler(G2, L2, sqrtW, inst.dd).run(normals, uniforms, return_probs=True)
+        mb = (runtime.to_host(pb) * np.arange(inst.dd)).sum(axis=2).mean(axis=1)
+        rel = float(np.max(np.abs(ma - mb) / ma))
+        results[tau] = (1 - w_learned / w_pruned, 1 - c_learned / c_pruned, rel)
+        print(f"tau {tau:g}: network's saving over the pruned plain form: hafnian work {results[tau][0]:.3%}, "
+              f"contraction work {results[tau][1]:.3%}, ranks below kept at "
+              f"{sum(r < k for r, k in zip(site_chi, kept))} of {len(kept)} cuts; first-order marginal "
+              f"change {rel:.2e}", flush=True)
+        del G2, L2, pb
+    assert results[learned.TAU_CHI][2] < 1e-6            # the 1e-6 bar holds at the default tolerance
+    assert results[learned.TAU_CHI_TRAIN][2] < 1e-4      # the looser tolerance costs ~1e-5

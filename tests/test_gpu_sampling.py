@@ -222,9 +222,10 @@ def test_site_contract_bf16x6_matches_matmul():
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(9)
-    # the last shape has k >= 2048 (128-column tiles) and a last tile past the padded columns
+    # (200, 2100, ...) has k >= 2048 (128-column tiles) and a last tile past the padded columns;
+    # (96, 9000, ...) has k > 4096, so it runs in k segments added in FP64
     for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
-                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2)):
+                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2), (96, 9000, 20, 3, 0)):
         scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
         scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
         pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
@@ -293,9 +294,10 @@ def test_site_contract_fp16x3_matches_matmul():
     from bosonsupremacy_amd import runtime
     ctx = runtime.context()
     rng = np.random.default_rng(10)
-    # the last shape has k >= 2048 (128-column tiles) and a last tile past the padded columns
+    # (200, 2100, ...) has k >= 2048 (128-column tiles) and a last tile past the padded columns;
+    # (96, 9000, ...) has k > 4096, so it runs in k segments added in FP64
     for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
-                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2)):
+                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2), (96, 9000, 20, 3, 0)):
         scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
         scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
         pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
@@ -309,8 +311,9 @@ def test_site_contract_fp16x3_matches_matmul():
         bound = (chi_p * np.outer(rs, cs)).reshape(n, chi, d)
         got = _fp16x3_contract(ctx, pre, lam, G)
         ratio = (np.abs(got - want) / bound).max()
-        print(f"fp16x3 {n}x{chi_p}x{chi * d} spread {spread}: max err / (K rowscale colscale) = {ratio:.2e}",
-              flush=True)
+        typical = np.median(np.abs(got - want) / np.sqrt(chi_p * np.outer(rs, cs)).reshape(n, chi, d))
+        print(f"fp16x3 {n}x{chi_p}x{chi * d} spread {spread}: max err / (K rowscale colscale) = {ratio:.2e}, "
+              f"median err / (sqrt(K) rowscale colscale) = {typical:.2e}", flush=True)
         assert ratio <= 2.0 ** -19, (n, chi_p, chi, d)
 
 
