@@ -60,6 +60,9 @@ _PROTOTYPES = {
     "bsmps_sigma_select": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_dense": ([c_void_p, c_void_p, c_int64, c_int, c_void_p], c_int),
     "bsmps_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_void_p], c_int),
+    "bsmps_loop_hafnian_dense": ([c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p], c_int),
+    "bsmps_loop_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p,
+                                    c_void_p], c_int),
     "bsmps_gamma_site": ([c_void_p, ctypes.POINTER(GammaSiteArgs), c_void_p], c_int),
     "bsmps_kron_cuts": ([c_void_p, c_int, c_int32_p, c_int, c_int, c_void_p, c_int64, c_int, c_double,
                          c_void_p, c_void_p, c_void_p], c_int),

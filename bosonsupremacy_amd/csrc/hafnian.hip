@@ -722,9 +722,11 @@ __device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int
     }
 }
 
-// One tree level of the upper phase: workgroup = (item (op, entry) of this level, block of 64
-// hafnians), placed by xcd_work.
-template <int M>
+// One tree level of the upper phase, for the RIGHT steps of one node degree D: workgroup =
+// (item (op, entry) of this level and degree, block of 64 hafnians), placed by xcd_work.  Each
+// degree has its own kernel, so a launch of low-degree items is not held to the registers of the
+// degree-M items (202 VGPRs at M = 11, two waves per SIMD).
+template <int M, int D>
 __global__ __launch_bounds__(kWave) void haf_bfs_level_kernel(BfsData bd, const BfsOp* __restrict__ ops,
                                                               const int2* __restrict__ items, int n_items,
                                                               int n_hblocks) {
@@ -734,14 +736,7 @@ __global__ __launch_bounds__(kWave) void haf_bfs_level_kernel(BfsData bd, const 
     if (h >= bd.n_h) return;
     const int2 it = items[item];
     const BfsOp op = ops[it.x];
-    // the plan only makes degrees 0, 1, 3, 7, 15 and M (checked on the host); M first, since
-    // it can coincide with 7 or 15
-    if (op.D == M) bfs_item<M, M, M>(bd, op, it.y, h);
-    else if (op.D == 0) bfs_item<M, 0, cmin(M, 1)>(bd, op, it.y, h);
-    else if (op.D == 1) bfs_item<M, 1, cmin(M, 3)>(bd, op, it.y, h);
-    else if (op.D == 3) { if constexpr (M > 3) bfs_item<M, 3, cmin(M, 7)>(bd, op, it.y, h); }
-    else if (op.D == 7) { if constexpr (M > 7) bfs_item<M, 7, cmin(M, 15)>(bd, op, it.y, h); }
-    else if (op.D == 15) { if constexpr (M > 15) bfs_item<M, 15, cmin(M, 31)>(bd, op, it.y, h); }
+    bfs_item<M, D, (D == M ? M : cmin(M, 2 * D + 1))>(bd, op, it.y, h);
 }
 
 template <int OFF>
@@ -827,6 +822,224 @@ __global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0
     const int64_t item = item0 + h;
     if constexpr (std::is_same<S, GammaSource>::value) src.store_dec(src.decode(item), v);
     else src.store(item, v);
+}
+
+// ---------------------------------------------------------------------------------------
+// Loop hafnians: a vertex may also be matched to itself, with weight loops[i].  The reference
+// computes plain hafnians only (its pure state has zero mean); this is the same truncated-
+// polynomial recursion with one more polynomial per row, l(J): the weight of the chains of
+// matching and pairing edges that start at a looped vertex of an already contracted pair and
+// end at row J (l(J) = loops[J] at the root).  Contracting the pivot pair (0, 1) closes a
+// cycle or a chain looped at both ends there, e = g + x g (b(1,0) + l(0) l(1)), and extends
+// the loop chains through it, l'(J) = l(J) + x (l(0) b(J,1) + l(1) b(J,0)); b is updated as in
+// the plain recursion.  An odd size is padded with one vertex of loop weight 1 and no edges.
+//
+// One lane per matrix walks the recursion tree depth first.  Every node of level L has the
+// rows 2L..2M-1, so one node per level is stored: its entries, loop polynomials and g (all
+// M + 1 coefficients, no term skipped) and the value of its first child, slot-major
+// [slot][matrix] in scratch so the lanes of a wave access consecutive matrices.  The terms are
+// added in the same order as the C restatement in the oracle, so the results are identical to
+// it; with every loop weight zero the added terms are exact zeros and the value is the plain
+// recursion's.
+// ---------------------------------------------------------------------------------------
+
+constexpr int kLoopMaxM = BSMPS_MAX_LOOP_HAFNIAN_SIZE / 2;
+
+template <int M>
+struct LoopLayout {
+    static constexpr int NP = M + 1;
+    __host__ __device__ static constexpr int rows(int L) { return 2 * M - 2 * L; }
+    __host__ __device__ static constexpr int entries(int L) { return rows(L) * (rows(L) - 1) / 2; }
+    __host__ __device__ static constexpr int level_slots(int L) { return (entries(L) + rows(L) + 1) * NP + 1; }
+    __host__ __device__ static constexpr int off(int L) {
+        int o = 0;
+        for (int i = 0; i < L; ++i) o += level_slots(i);
+        return o;
+    }
+    __host__ __device__ static constexpr int total() { return off(M + 1); }
+    // slots relative to off(L)
+    __host__ __device__ static constexpr int b(int J, int K, int t) { return tri(J, K) * NP + t; }
+    __host__ __device__ static constexpr int l(int L, int J, int t) { return (entries(L) + J) * NP + t; }
+    __host__ __device__ static constexpr int g(int L, int t) { return (entries(L) + rows(L)) * NP + t; }
+    __host__ __device__ static constexpr int first(int L) { return (entries(L) + rows(L) + 1) * NP; }
+};
+
+struct LoopData {
+    double* data;    // complex slots, slot-major
+    int64_t stride;  // matrices per slot (chunk capacity)
+    int64_t n_h;     // matrices in this chunk
+};
+
+template <int M>
+struct LoopNode {
+    using Lay = LoopLayout<M>;
+    static constexpr int NP = M + 1;
+    double* D;
+    int64_t st, h;
+    __device__ cd ld(int o, int s) const { return ld_cd(D, (int64_t)(o + s) * st + h); }
+    __device__ void put(int o, int s, cd v) const { st_cd(D, (int64_t)(o + s) * st + h, v); }
+
+    // left child of the level-L node: rows 2.. of it, same g
+    __device__ void left(int L) const {
+        const int o = Lay::off(L), o1 = Lay::off(L + 1), s1 = Lay::rows(L) - 2;
+        for (int j = 1; j < s1; ++j)
+            for (int k = 0; k < j; ++k)
+#pragma unroll
+                for (int t = 0; t < NP; ++t) put(o1, Lay::b(j, k, t), ld(o, Lay::b(j + 2, k + 2, t)));
+        for (int j = 0; j < s1; ++j)
+#pragma unroll
+            for (int t = 0; t < NP; ++t) put(o1, Lay::l(L + 1, j, t), ld(o, Lay::l(L, j + 2, t)));
+#pragma unroll
+        for (int t = 0; t < NP; ++t) put(o1, Lay::g(L + 1, t), ld(o, Lay::g(L, t)));
+    }
+
+    // right child of the level-L node: pair (0, 1) contracted
+    __device__ void right(int L) const {
+        const int o = Lay::off(L), o1 = Lay::off(L + 1), s1 = Lay::rows(L) - 2;
+        cd l0[M], l1[M];
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+            l0[u] = ld(o, Lay::l(L, 0, u));
+            l1[u] = ld(o, Lay::l(L, 1, u));
+        }
+        {
+            // q[v] = b(1,0)[v] + sum_p l(0)[p] l(1)[v-p] (ascending p); e = g + x g q
+            cd q[M], g[NP];
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                cd acc = cmul(l0[0], l1[v]);
+#pragma unroll
+                for (int p = 1; p <= v; ++p) acc = cadd(acc, cmul(l0[p], l1[v - p]));
+                q[v] = cadd(ld(o, Lay::b(1, 0, v)), acc);
+            }
+#pragma unroll
+            for (int t = 0; t < NP; ++t) g[t] = ld(o, Lay::g(L, t));
+#pragma unroll
+            for (int t = 0; t < NP; ++t) {
+                cd e = g[t];
+#pragma unroll
+                for (int u = 0; u < t; ++u) e = cadd(e, cmul(g[u], q[t - 1 - u]));
+                put(o1, Lay::g(L + 1, t), e);
+            }
+        }
+        for (int j = 1; j < s1; ++j) {
+            cd X[M], W[M];
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                X[u] = ld(o, Lay::b(j + 2, 0, u));
+                W[u] = ld(o, Lay::b(j + 2, 1, u));
+            }
+            for (int k = 0; k < j; ++k) {
+                cd Y[M], Z[M];
+#pragma unroll
+                for (int u = 0; u < M; ++u) {
+                    Y[u] = ld(o, Lay::b(k + 2, 1, u));
+                    Z[u] = ld(o, Lay::b(k + 2, 0, u));
+                }
+#pragma unroll
+                for (int t = 0; t < NP; ++t) {
+                    cd acc = ld(o, Lay::b(j + 2, k + 2, t));
+#pragma unroll
+                    for (int u = 0; u < t; ++u) {
+                        const int v = t - 1 - u;
+                        acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+                    }
+                    put(o1, Lay::b(j, k, t), acc);
+                }
+            }
+        }
+        for (int j = 0; j < s1; ++j) {
+            cd B0[M], B1[M];
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                B0[u] = ld(o, Lay::b(j + 2, 0, u));
+                B1[u] = ld(o, Lay::b(j + 2, 1, u));
+            }
+#pragma unroll
+            for (int t = 0; t < NP; ++t) {
+                cd acc = ld(o, Lay::l(L, j + 2, t));
+#pragma unroll
+                for (int u = 0; u < t; ++u) {
+                    const int v = t - 1 - u;
+                    acc = cadd(acc, cadd(cmul(l0[u], B1[v]), cmul(l1[u], B0[v])));
+                }
+                put(o1, Lay::l(L + 1, j, t), acc);
+            }
+        }
+    }
+};
+
+// loops: per matrix [n_batch][n_sel] when loop_stride = n_sel (explicit matrices), else a
+// vector over the rows of the matrix the targets index (loop weight of row J = loops[t[J]]).
+template <int M, class S>
+__global__ __launch_bounds__(kWave) void haf_loop_kernel(S src, const double* __restrict__ loops, int loop_stride,
+                                                         int n_sel, int64_t item0, LoopData lp) {
+    using Lay = LoopLayout<M>;
+    constexpr int NP = M + 1, NE = 2 * M;
+    const int64_t h = (int64_t)blockIdx.x * kWave + threadIdx.x;
+    if (h >= lp.n_h) return;
+    const int64_t item = item0 + h;
+    const LoopNode<M> nd{lp.data, lp.stride, h};
+    const double* A = src.base(item);
+    const int lda = src.ldim();
+    for (int J = 0; J < NE; ++J) {  // rows J >= n_sel pad an odd size
+        int tj = 0;
+        if (J < n_sel) src.fetch(item, J, tj);
+        for (int K = 0; K < J; ++K) {
+            int tk = 0;
+            src.fetch(item, K, tk);
+            nd.put(0, Lay::b(J, K, 0), J < n_sel ? ld_cd(A, (int64_t)tj * lda + tk) : czero());
+#pragma unroll
+            for (int c = 1; c < NP; ++c) nd.put(0, Lay::b(J, K, c), czero());
+        }
+        const cd lw = J < n_sel ? ld_cd(loops, loop_stride ? item * loop_stride + J : (int64_t)tj) : cd{1.0, 0.0};
+        nd.put(0, Lay::l(0, J, 0), lw);
+#pragma unroll
+        for (int c = 1; c < NP; ++c) nd.put(0, Lay::l(0, J, c), czero());
+    }
+#pragma unroll
+    for (int c = 0; c < NP; ++c) nd.put(0, Lay::g(0, c), c == 0 ? cd{1.0, 0.0} : czero());
+    // depth-first walk; state of level L in bits 2L..2L+1: 0 fresh, 1 left child taken,
+    // 2 right child taken.  A child returns its value in rv; w is the current node's sign.
+    uint32_t states = 0;
+    int L = 0;
+    double w = 1.0;
+    bool ret = false;
+    cd rv = czero();
+    for (;;) {
+        const uint32_t stl = (states >> (2 * L)) & 3u;
+        if (ret) {
+            if (stl == 1u) {  // left child done: keep its value, take the right child
+                nd.put(Lay::off(L), Lay::first(L), rv);
+                w = -w;
+                ret = false;
+            } else {  // right child done: h + r, returned to the parent
+                rv = cadd(nd.ld(Lay::off(L), Lay::first(L)), rv);
+                if (L == 0) break;
+                --L;
+                continue;
+            }
+        }
+        if (L == M) {  // leaf: w g[M]
+            rv = cmul(cd{w, 0.0}, nd.ld(Lay::off(M), Lay::g(M, M)));
+            ret = true;
+            --L;
+            continue;
+        }
+        if (stl == 0u) {
+            nd.left(L);
+            states |= 1u << (2 * L);
+            ++L;
+            states &= ~(3u << (2 * L));
+            w = -w;
+        } else {
+            nd.right(L);
+            states = (states & ~(3u << (2 * L))) | (2u << (2 * L));
+            ++L;
+            states &= ~(3u << (2 * L));
+        }
+    }
+    src.store(item, rv);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -957,7 +1170,7 @@ struct FrontierPlan {
     int a_off, rootg;
     BfsOp* d_ops;
     int2* d_items;                   // (op, entry or -1), grouped by level
-    std::vector<int> level_first, level_count;
+    std::vector<int> level_first, level_count, level_degree;  // item groups of one (level, degree)
     int4* d_front;
     int n_front;
     int* d_tlist;
@@ -986,21 +1199,31 @@ int get_fplan(int device, int m, FrontierPlan** out) {
         p.slots = fb.slots;
         // work items of each level: every child entry of every RIGHT step, then its g update
         std::vector<int2> items;
+        // (a level's items are independent, so its degree groups may run one after another)
+        const int degrees[6] = {m, 0, 1, 3, 7, 15};
         for (int l = 0; l < m; ++l) {
-            const int first = (int)items.size();
-            for (size_t o = 0; o < fb.ops.size(); ++o) {
-                if (fb.ops[o].level != l) continue;
-                const int s0 = 2 * m - 2 * (l + 1);
-                for (int e = 0; e < s0 * (s0 - 1) / 2; ++e) items.push_back(make_int2((int)o, e));
-                items.push_back(make_int2((int)o, -1));
-            }
-            if ((int)items.size() > first) {
-                p.level_first.push_back(first);
-                p.level_count.push_back((int)items.size() - first);
+            for (int di = 0; di < 6; ++di) {
+                const int D = degrees[di];
+                if (di > 0 && D == m) continue;  // M coincides with 7 or 15
+                const int first = (int)items.size();
+                for (size_t o = 0; o < fb.ops.size(); ++o) {
+                    if (fb.ops[o].level != l || fb.ops[o].D != D) continue;
+                    const int s0 = 2 * m - 2 * (l + 1);
+                    for (int e = 0; e < s0 * (s0 - 1) / 2; ++e) items.push_back(make_int2((int)o, e));
+                    items.push_back(make_int2((int)o, -1));
+                }
+                if ((int)items.size() > first) {
+                    p.level_first.push_back(first);
+                    p.level_count.push_back((int)items.size() - first);
+                    p.level_degree.push_back(D);
+                }
             }
         }
         for (const BfsOp& op : fb.ops)
             if (!(op.D == 0 || op.D == 1 || op.D == 3 || op.D == 7 || op.D == 15 || op.D == m)) p.ok = false;
+        size_t n_level_items = 0;
+        for (int c : p.level_count) n_level_items += (size_t)c;
+        if (n_level_items != items.size()) p.ok = false;
         p.n_front = (int)fb.front.size();
         std::map<std::pair<int, int>, std::vector<int>> by_type;
         for (int f = 0; f < p.n_front; ++f) {
@@ -1037,6 +1260,27 @@ int get_fplan(int device, int m, FrontierPlan** out) {
         it = g_fplans.emplace(key, p).first;
     }
     *out = &it->second;
+    return BSMPS_OK;
+}
+
+template <int M>
+int launch_level_m(hipStream_t st, int D, int n_items, int n_hblocks, const BfsData& bd, const BfsOp* ops,
+                   const int2* items) {
+    const dim3 grid((unsigned)xcd_grid(n_items, n_hblocks));
+#define BSMPS_LEVEL(DD) \
+    hipLaunchKernelGGL((haf_bfs_level_kernel<M, DD>), grid, dim3(kWave), 0, st, bd, ops, items, n_items, n_hblocks)
+    if (D == M) BSMPS_LEVEL(M);
+    else if (D == 0) BSMPS_LEVEL(0);
+    else if (D == 1) BSMPS_LEVEL(1);
+    else if (D == 3 && M > 3) BSMPS_LEVEL((M > 3 ? 3 : M));
+    else if (D == 7 && M > 7) BSMPS_LEVEL((M > 7 ? 7 : M));
+    else if (D == 15 && M > 15) BSMPS_LEVEL((M > 15 ? 15 : M));
+    else {
+        bsmps_set_error("no level kernel for size %d, degree %d", 2 * M, D);
+        return BSMPS_ERR_UNSUPPORTED;
+    }
+#undef BSMPS_LEVEL
+    BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
 
@@ -1096,13 +1340,9 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
         hipLaunchKernelGGL((haf_bfs_prep_kernel<MM, S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, bd, \
                            fp->rootg);                                                                        \
         BSMPS_HIP(hipGetLastError());                                                                         \
-        for (size_t l = 0; l < fp->level_first.size(); ++l) {                                                 \
-            hipLaunchKernelGGL((haf_bfs_level_kernel<MM>),                                                    \
-                               dim3((unsigned)xcd_grid(fp->level_count[l], (int)hblocks)), dim3(kWave), 0,    \
-                               ctx->stream, bd, fp->d_ops, fp->d_items + fp->level_first[l],                  \
-                               fp->level_count[l], (int)hblocks);                                             \
-            BSMPS_HIP(hipGetLastError());                                                                     \
-        }                                                                                                     \
+        for (size_t l = 0; l < fp->level_first.size(); ++l)                                                   \
+            BSMPS_TRY(launch_level_m<MM>(ctx->stream, fp->level_degree[l], fp->level_count[l], (int)hblocks, bd, \
+                                         fp->d_ops, fp->d_items + fp->level_first[l]));                       \
         for (size_t t = 0; t < fp->types.size(); ++t)                                                         \
             BSMPS_TRY(launch_subtrees_m<MM>(ctx->stream, fp->types[t].x, fp->types[t].y, fp->type_count[t],   \
                                             (int)hblocks, bd, fp->d_front, fp->d_tlist + fp->type_first[t],   \
@@ -1172,6 +1412,50 @@ int fill_complex(bsmps_ctx* ctx, double* out, int64_t n, double re, double im) {
     return BSMPS_OK;
 }
 
+template <int M, class S>
+int launch_loop_m(bsmps_ctx* ctx, const S& src, const double* loops, int loop_stride, int n_sel) {
+    using Lay = LoopLayout<M>;
+    const size_t per_h = sizeof(cd) * (size_t)Lay::total();
+    int64_t H = (int64_t)(ctx->split_chunk_bytes / per_h);
+    H = std::max<int64_t>(kWave, H / kWave * kWave);
+    H = std::min<int64_t>(H, (src.n_items + kWave - 1) / kWave * kWave);
+    void* scratch = nullptr;
+    BSMPS_TRY(bsmps_ctx_scratch(ctx, per_h * (size_t)H, &scratch));
+    for (int64_t c0 = 0; c0 < src.n_items; c0 += H) {
+        const int64_t nh = std::min<int64_t>(H, src.n_items - c0);
+        LoopData lp{reinterpret_cast<double*>(scratch), H, nh};
+        hipLaunchKernelGGL((haf_loop_kernel<M, S>), dim3((unsigned)((nh + kWave - 1) / kWave)), dim3(kWave), 0,
+                           ctx->stream, src, loops, loop_stride, n_sel, c0, lp);
+        BSMPS_HIP(hipGetLastError());
+    }
+    return BSMPS_OK;
+}
+
+template <class S>
+int launch_loop(bsmps_ctx* ctx, const S& src, const double* loops, int loop_stride, int n_sel) {
+    switch ((n_sel + 1) / 2) {
+#define BSMPS_LOOP_CASE(MM) \
+    case MM:                \
+        return launch_loop_m<MM, S>(ctx, src, loops, loop_stride, n_sel);
+        BSMPS_LOOP_CASE(1)
+        BSMPS_LOOP_CASE(2)
+        BSMPS_LOOP_CASE(3)
+        BSMPS_LOOP_CASE(4)
+        BSMPS_LOOP_CASE(5)
+        BSMPS_LOOP_CASE(6)
+        BSMPS_LOOP_CASE(7)
+        BSMPS_LOOP_CASE(8)
+        BSMPS_LOOP_CASE(9)
+        BSMPS_LOOP_CASE(10)
+        BSMPS_LOOP_CASE(11)
+        BSMPS_LOOP_CASE(12)
+#undef BSMPS_LOOP_CASE
+        default:
+            bsmps_set_error("loop hafnian of size %d: sizes 1..%d are supported", n_sel, BSMPS_MAX_LOOP_HAFNIAN_SIZE);
+            return BSMPS_ERR_INVALID;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -1228,6 +1512,44 @@ extern "C" int bsmps_hafnian_targets(bsmps_ctx* ctx, const double* sigma, int ld
     if (n_select % 2) return fill_complex(ctx, out, n_batch, 0.0, 0.0);
     TargetsSource s{sigma, ld, targets, n_select, den, out, n_batch};
     return launch_hafnians(ctx, s, n_select);
+}
+
+extern "C" int bsmps_loop_hafnian_dense(bsmps_ctx* ctx, const double* A, const double* loops, int64_t n_batch,
+                                        int n, double* out) {
+    if (!loops) return bsmps_hafnian_dense(ctx, A, n_batch, n, out);
+    if (!ctx || n_batch < 0 || n < 0 || n > BSMPS_MAX_LOOP_HAFNIAN_SIZE) {
+        bsmps_set_error("bsmps_loop_hafnian_dense: invalid arguments (n = %d, max %d)", n,
+                        BSMPS_MAX_LOOP_HAFNIAN_SIZE);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n_batch == 0) return BSMPS_OK;
+    if (!out || (n > 0 && !A)) {
+        bsmps_set_error("bsmps_loop_hafnian_dense: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    if (n == 0) return fill_complex(ctx, out, n_batch, 1.0, 0.0);
+    DenseSource s{A, n, out, n_batch};
+    return launch_loop(ctx, s, loops, n, n);
+}
+
+extern "C" int bsmps_loop_hafnian_targets(bsmps_ctx* ctx, const double* sigma, int ld, const double* loops,
+                                          const int32_t* targets, int64_t n_batch, int n_select, const double* den,
+                                          double* out) {
+    if (!loops) return bsmps_hafnian_targets(ctx, sigma, ld, targets, n_batch, n_select, den, out);
+    if (!ctx || n_batch < 0 || n_select < 0 || n_select > BSMPS_MAX_LOOP_HAFNIAN_SIZE || ld <= 0) {
+        bsmps_set_error("bsmps_loop_hafnian_targets: invalid arguments (n_select = %d, max %d)", n_select,
+                        BSMPS_MAX_LOOP_HAFNIAN_SIZE);
+        return BSMPS_ERR_INVALID;
+    }
+    if (n_batch == 0) return BSMPS_OK;
+    if (!out || !sigma || (n_select > 0 && !targets)) {
+        bsmps_set_error("bsmps_loop_hafnian_targets: null pointer");
+        return BSMPS_ERR_INVALID;
+    }
+    // the empty matrix has no loops: its value (1, then the epilogue) is the plain hafnian's
+    if (n_select == 0) return bsmps_hafnian_targets(ctx, sigma, ld, targets, n_batch, n_select, den, out);
+    TargetsSource s{sigma, ld, targets, n_select, den, out, n_batch};
+    return launch_loop(ctx, s, loops, 0, n_select);
 }
 
 extern "C" int bsmps_gamma_site(bsmps_ctx* ctx, const bsmps_gamma_site_args* a, double* gamma) {

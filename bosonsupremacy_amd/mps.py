@@ -82,32 +82,47 @@ def Sigma_select(Sigma, target):
     return runtime.to_host(out)
 
 
-def hafnian(A):
-    """Batched hafnian of A [B, n, n] (MPS_cpu.py:117-187), complex128."""
+def hafnian(A, loops=None):
+    """Batched hafnian of A [B, n, n] (MPS_cpu.py:117-187), complex128.
+
+    ``loops`` (optional, [B, n] or [n]): loop weights, for the loop hafnian (a vertex may be
+    matched to itself); the reference has none, and None is its plain hafnian."""
     ctx = runtime.context()
     Ad = _as_complex_dev(A)
     B, n, n2 = Ad.shape
     if n != n2:
         raise ValueError("Matrix must be square")
     out = torch.empty((B,), dtype=torch.complex128, device=ctx.device)
-    ctx.call("bsmps_hafnian_dense", runtime.ptr(Ad), B, n, runtime.ptr(out))
+    if loops is None:
+        ctx.call("bsmps_hafnian_dense", runtime.ptr(Ad), B, n, runtime.ptr(out))
+    else:
+        Ld = _as_complex_dev(np.broadcast_to(runtime.host(loops).astype(np.complex128), (B, n)).copy())
+        ctx.call("bsmps_loop_hafnian_dense", runtime.ptr(Ad), runtime.ptr(Ld), B, n, runtime.ptr(out))
     return runtime.to_host(out)
 
 
-def A_elem(Sigma, target, denominator, max_memory_in_gb=0.5):
+def A_elem(Sigma, target, denominator, max_memory_in_gb=0.5, loops=None):
     """haf(Sigma[t][:, t]) / denominator for each target row (MPS_cpu.py:334-355).
 
     Returns (values, haf_time, sigma_time) like the reference; the gather is fused into the
     hafnian kernel, so sigma_time is 0.  ``max_memory_in_gb`` is accepted for signature
-    compatibility; the GPU path never materialises the gathered matrices."""
+    compatibility; the GPU path never materialises the gathered matrices.  ``loops``
+    (optional, one weight per row of Sigma, gathered like Sigma): the loop hafnian instead."""
     ctx = runtime.context()
     S = _as_complex_dev(Sigma)
     T = runtime.to_device(np.ascontiguousarray(target, dtype=np.int32))
     den = runtime.to_device(np.ascontiguousarray(np.broadcast_to(denominator, (T.shape[0],)), dtype=np.float64))
     out = torch.empty((T.shape[0],), dtype=torch.complex128, device=ctx.device)
     t0 = time.time()
-    ctx.call("bsmps_hafnian_targets", runtime.ptr(S), S.shape[0], runtime.ptr(T), T.shape[0], T.shape[1],
-             runtime.ptr(den), runtime.ptr(out))
+    if loops is None:
+        ctx.call("bsmps_hafnian_targets", runtime.ptr(S), S.shape[0], runtime.ptr(T), T.shape[0], T.shape[1],
+                 runtime.ptr(den), runtime.ptr(out))
+    else:
+        Ld = _as_complex_dev(loops)
+        if Ld.shape != (S.shape[0],):
+            raise ValueError("loops must hold one weight per row of Sigma")
+        ctx.call("bsmps_loop_hafnian_targets", runtime.ptr(S), S.shape[0], runtime.ptr(Ld), runtime.ptr(T),
+                 T.shape[0], T.shape[1], runtime.ptr(den), runtime.ptr(out))
     res = runtime.to_host(out)
     return res, time.time() - t0, 0.0
 

@@ -33,6 +33,8 @@ enum {
 
 /* Largest hafnian matrix size (rows) the hafnian entry points accept. */
 #define BSMPS_MAX_HAFNIAN_SIZE 34
+/* Largest matrix size (rows) the loop-hafnian entry points accept with a loop-weight vector. */
+#define BSMPS_MAX_LOOP_HAFNIAN_SIZE 24
 
 typedef struct bsmps_ctx bsmps_ctx;
 
@@ -90,6 +92,20 @@ int bsmps_hafnian_dense(bsmps_ctx *ctx, const double *A, int64_t n_batch, int n,
  * (no division).  Division is NumPy's complex/real: both parts times 1/den. */
 int bsmps_hafnian_targets(bsmps_ctx *ctx, const double *sigma, int ld, const int32_t *targets,
                           int64_t n_batch, int n_select, const double *den, double *out);
+
+/* The hafnian entry points with an optional loop-weight vector (a vertex may be matched to
+ * itself with weight loops[i]).  The reference computes plain hafnians only (MPS_cpu.py:117-140:
+ * its pure state has zero mean), so loops = NULL is the parity case and calls the plain entry
+ * point (bit-exact to the reference).  With loops, the reference recursion extended by one loop
+ * polynomial per row (sizes 1..BSMPS_MAX_LOOP_HAFNIAN_SIZE; an odd size is padded with a vertex
+ * of loop weight 1 and no edges); an all-zero vector gives the plain recursion's value.
+ * Dense: loops [n_batch][n], one vector per matrix.  Targets: loops [ld], a weight per row of
+ * sigma, gathered through the targets like sigma (row r of a matrix has weight loops[t_r]). */
+int bsmps_loop_hafnian_dense(bsmps_ctx *ctx, const double *A, const double *loops, int64_t n_batch, int n,
+                             double *out);
+int bsmps_loop_hafnian_targets(bsmps_ctx *ctx, const double *sigma, int ld, const double *loops,
+                               const int32_t *targets, int64_t n_batch, int n_select, const double *den,
+                               double *out);
 
 /* One site tensor Gamma_k, the j / photon-total / chunk loops of MPS_cpu.py:402-520
  * (first site :402-428, last site :430-459, middle sites :461-520; GPU path

@@ -29,6 +29,10 @@ def _load():
         lib.oracle_hafnian_targets.restype = None
         lib.oracle_hafnian_dense.argtypes = [dp, ctypes.c_long, ctypes.c_int, dp]
         lib.oracle_hafnian_dense.restype = None
+        lib.oracle_loop_hafnian_targets.argtypes = [dp, ctypes.c_int, dp, ip, ctypes.c_long, ctypes.c_int, dp]
+        lib.oracle_loop_hafnian_targets.restype = None
+        lib.oracle_loop_hafnian_dense.argtypes = [dp, dp, ctypes.c_long, ctypes.c_int, dp]
+        lib.oracle_loop_hafnian_dense.restype = None
         _lib = lib
     return _lib
 
@@ -54,4 +58,34 @@ def hafnian_dense(A):
     out = np.zeros(A.shape[0], dtype=np.complex128)
     lib.oracle_hafnian_dense(A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), A.shape[0],
                              A.shape[1], out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def loop_hafnian_dense(A, loops):
+    """Loop hafnian of each matrix of A [B, n, n] with loop weights loops [B, n] (the plain
+    recursion of MPS_cpu.py:143-187 extended by one loop polynomial per row; an odd n is padded
+    with a vertex of loop weight 1 and no edges)."""
+    lib = _load()
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    L = np.ascontiguousarray(np.broadcast_to(loops, A.shape[:2]), dtype=np.complex128)
+    out = np.zeros(A.shape[0], dtype=np.complex128)
+    lib.oracle_loop_hafnian_dense(_dp(A), _dp(L), A.shape[0], A.shape[1], _dp(out))
+    return out
+
+
+def loop_hafnian_targets(Sigma, loops, targets):
+    """lhaf(Sigma[t][:, t]) with loop weights loops[t] for each row t of ``targets``."""
+    lib = _load()
+    S = np.ascontiguousarray(Sigma, dtype=np.complex128)
+    L = np.ascontiguousarray(loops, dtype=np.complex128)
+    T = np.ascontiguousarray(targets, dtype=np.int32)
+    if T.ndim != 2 or L.shape != (S.shape[0],):
+        raise ValueError("targets must be 2-D and loops a vector over Sigma's rows")
+    out = np.zeros(T.shape[0], dtype=np.complex128)
+    lib.oracle_loop_hafnian_targets(_dp(S), S.shape[0], _dp(L), T.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    T.shape[0], T.shape[1], _dp(out))
     return out
