@@ -604,6 +604,8 @@ def main():
                     help="FIRST-LAST: construct-and-sample sites FIRST..LAST of --workload site-major (chain.Chain), "
                          "FP64 drawing and FP16x3 replaying its draws; resumes from --chain-resume")
     ap.add_argument("--chain-samples", type=int, default=200)
+    ap.add_argument("--chain-forms", default="fp64,fp16x3",
+                    help="precisions walked by the chain, the first draws (e.g. fp64,fp16x3,bf16x6,fp32)")
     ap.add_argument("--chain-resume", default=None, help="checkpoint written by the previous part of the chain")
     ap.add_argument("--chain-save", default=None, help="where to write this part's checkpoint")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -690,7 +692,7 @@ def chain_run(args):
     if args.chain_resume:   # every part of a chain walks the forms its first part chose
         forms = chain.Chain.forms_of(args.chain_resume)
     else:
-        forms = [chain.Form("fp64"), chain.Form("fp16x3", "fp16x3")]
+        forms = [chain.Form(p, p) for p in args.chain_forms.split(",")]
         if not same_plan and not args.no_learned:
             forms.append(chain.Form("learned_fp16x3", "fp16x3", site_chi=sc, site_d=sd))
     c = chain.Chain(inst.sq_cov, inst.cov, inst.d, inst.chi, inst.dd, args.chain_samples, seed=11, forms=forms,

@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void site_gemm_bf16x3_kernel(int64_t n, int ch
 // imaginary accumulator.
 // ---------------------------------------------------------------------------------------
 constexpr int X6_TM = 128, X6_TN = 64, X6_KC = 32;
-// FP16x3 contractions with more than kSplitKMin values of k run in k segments of kSplitKSeg
+// Split-form contractions with more than kSplitKMin values of k run in k segments of kSplitKSeg
 // (see site_gemm_split_kernel)
 constexpr int64_t kSplitKMin = 4096, kSplitKSeg = 2048;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -1640,6 +1640,32 @@ extern "C" int bsmps_split_rows_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, 
     return BSMPS_OK;
 }
 
+// The split-form contraction of either format.  Long k runs in segments of kSplitKSeg values
+// with the partials in the context workspace and an FP64 combine in segment order.
+template <class F>
+int launch_split_contract(bsmps_ctx* ctx, int64_t n, int64_t ncols, int64_t kpad, int64_t ncols_pad, int64_t gx,
+                          int64_t gy, const typename F::E* apl, const float* rowscale, const typename F::E* bpl,
+                          const float* colscale, float* T) {
+    const int segs = kpad > kSplitKMin ? (int)((kpad + kSplitKSeg - 1) / kSplitKSeg) : 1;
+    float* out = T;
+    const int64_t count = n * ncols * 2;
+    if (segs > 1) {
+        void* ws = nullptr;
+        BSMPS_TRY(bsmps_ctx_workspace(ctx, sizeof(float) * (size_t)count * segs, &ws));
+        out = static_cast<float*>(ws);
+    }
+    hipLaunchKernelGGL(site_gemm_split_kernel<F>, dim3((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs), dim3(256),
+                       0, ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad, apl, rowscale, bpl, colscale, out,
+                       segs > 1 ? (int)kSplitKSeg : (int)kpad);
+    BSMPS_HIP(hipGetLastError());
+    if (segs > 1) {
+        hipLaunchKernelGGL(split_k_combine_kernel, dim3((unsigned)std::min<int64_t>((count + 255) / 256, 65536)),
+                           dim3(256), 0, ctx->stream, out, count, segs, T);
+        BSMPS_HIP(hipGetLastError());
+    }
+    return BSMPS_OK;
+}
+
 extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
                                           const void* pre_planes, const void* gamma_planes, float* T) {
     if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
@@ -1657,12 +1683,9 @@ extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_pre
         bsmps_set_error("bsmps_site_contract_bf16x6: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(site_gemm_split_kernel<Bf16x6Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy)), dim3(256), 0,
-                       ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
-                       reinterpret_cast<const __bf16*>(pre_planes), nullptr,
-                       reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T, (int)kpad);
-    BSMPS_HIP(hipGetLastError());
-    return BSMPS_OK;
+    return launch_split_contract<Bf16x6Fmt>(ctx, n, ncols, kpad, ncols_pad, gx, gy,
+                                            reinterpret_cast<const __bf16*>(pre_planes), nullptr,
+                                            reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T);
 }
 
 extern "C" int64_t bsmps_fp16x3_plane_bytes(int chi_prev, int chi, int d) {
@@ -1735,25 +1758,7 @@ extern "C" int bsmps_site_contract_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_pre
         bsmps_set_error("bsmps_site_contract_fp16x3: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    // long k: segments of kSplitKSeg values, partials in the context workspace, FP64 combine
-    const int segs = kpad > kSplitKMin ? (int)((kpad + kSplitKSeg - 1) / kSplitKSeg) : 1;
-    float* out = T;
-    const int64_t count = n * ncols * 2;
-    if (segs > 1) {
-        void* ws = nullptr;
-        BSMPS_TRY(bsmps_ctx_workspace(ctx, sizeof(float) * (size_t)count * segs, &ws));
-        out = static_cast<float*>(ws);
-    }
-    hipLaunchKernelGGL(site_gemm_split_kernel<Fp16x3Fmt>, dim3((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs),
-                       dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad,
-                       reinterpret_cast<const _Float16*>(pre_planes), rowscale,
-                       reinterpret_cast<const _Float16*>(gamma_planes), colscale, out,
-                       segs > 1 ? (int)kSplitKSeg : (int)kpad);
-    BSMPS_HIP(hipGetLastError());
-    if (segs > 1) {
-        hipLaunchKernelGGL(split_k_combine_kernel, dim3((unsigned)std::min<int64_t>((count + 255) / 256, 65536)),
-                           dim3(256), 0, ctx->stream, out, count, segs, T);
-        BSMPS_HIP(hipGetLastError());
-    }
-    return BSMPS_OK;
+    return launch_split_contract<Fp16x3Fmt>(ctx, n, ncols, kpad, ncols_pad, gx, gy,
+                                            reinterpret_cast<const _Float16*>(pre_planes), rowscale,
+                                            reinterpret_cast<const _Float16*>(gamma_planes), colscale, T);
 }
