@@ -686,19 +686,24 @@ __device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int
             Z[u] = need ? co(K, b0, u) : czero();
             W[u] = need ? co(J, b1, u) : czero();
         }
+        // The entry's own coefficients are all loaded before the first store: loads and stores
+        // share one buffer, so a load placed after a store cannot be moved above it, and each
+        // coefficient would then wait a full memory latency of its own.
+        cd acc[D2 + 1];
+#pragma unroll
+        for (int t = 1; t <= D2; ++t) acc[t] = (t <= tmax && t <= D) ? co(J, K, t) : czero();
         const int64_t dst = (int64_t)(op.pdst + e * op.dcap) * bd.stride + h;
 #pragma unroll
         for (int t = 1; t <= D2; ++t) {
             if (t <= tmax) {
                 // c(J,K)[t] = b(J,K)[t] + sum_u (b(J,b0)[u] b(K,b1)[v] + b(K,b0)[u] b(J,b1)[v]),
                 // v = t-1-u, ascending u (MPS_cpu.py:179-186)
-                cd acc = (t <= D) ? co(J, K, t) : czero();
 #pragma unroll
                 for (int u = 0; u <= DU; ++u) {
                     const int v = t - 1 - u;
-                    if (v >= 0 && v <= D && u <= D) acc = cadd(acc, cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
+                    if (v >= 0 && v <= D && u <= D) acc[t] = cadd(acc[t], cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
                 }
-                st_cd(bd.data, dst + (int64_t)(t - 1) * bd.stride, acc);
+                st_cd(bd.data, dst + (int64_t)(t - 1) * bd.stride, acc[t]);
             }
         }
     } else {
@@ -782,32 +787,34 @@ __global__ __launch_bounds__(kWave) void haf_subtree_kernel(BfsData bd, const in
     st_cd(vals, (int64_t)f * bd.stride + h, Node<M, K0, D, D>::run(A, P, g, w));
 }
 
-// Frontier split, last phase: add the subtree values in the order of the recursion tree
-// (comb: f >= 0 pushes subtree f, -1 adds the top two) and apply the item's epilogue.
+// Frontier split, last phase: add the subtree values in the order of the recursion tree and
+// apply the item's epilogue.  The combine program is postfix: c >= 0 pushes the value in
+// slot c, -1 replaces the top two by their sum (lower + upper, the recursion's left + right).
+// A long program is first cut into independent parts -- complete subtrees of the add tree,
+// each a contiguous stretch of the program -- that run in parallel (haf_combine_part_kernel,
+// one workgroup per (part, block of 64 hafnians)) and leave their sums in the slot of their
+// first value; the final kernel then runs the program of the tree above the parts.  Every sum
+// is formed from the same operands in the same order as in one pass over the whole program.
 constexpr int kCombineStack = 18;
 constexpr int kCombineBatch = 16;
-template <class S>
-__global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0, int64_t n_h, int64_t stride,
-                                                            const double* __restrict__ vals,
-                                                            const int* __restrict__ comb, int n_comb) {
-    __shared__ cd stack[kCombineStack * kWave];
-    const int lane = threadIdx.x;
-    const int64_t h = (int64_t)blockIdx.x * kWave + lane;
-    if (h >= n_h) return;
-    // the values pushed by the next kCombineBatch program steps are loaded before any is used,
-    // so a lane has that many loads in flight instead of one (the program is uniform across lanes)
+
+// Evaluates prog[0..n) for hafnian h; the values pushed by the next kCombineBatch steps are
+// loaded before any is used, so a lane has that many loads in flight instead of one (the
+// program is uniform across lanes).
+__device__ __forceinline__ cd combine_eval(const double* vals, int64_t stride, int64_t h,
+                                           const int* __restrict__ prog, int n, cd* stack, int lane) {
     int sp = 0;
-    for (int i0 = 0; i0 < n_comb; i0 += kCombineBatch) {
+    for (int i0 = 0; i0 < n; i0 += kCombineBatch) {
         cd v[kCombineBatch];
 #pragma unroll
         for (int u = 0; u < kCombineBatch; ++u) {
-            const int c = i0 + u < n_comb ? comb[i0 + u] : -1;
+            const int c = i0 + u < n ? prog[i0 + u] : -1;
             v[u] = c >= 0 ? ld_cd(vals, (int64_t)c * stride + h) : czero();
         }
 #pragma unroll
         for (int u = 0; u < kCombineBatch; ++u) {
-            if (i0 + u < n_comb) {
-                if (comb[i0 + u] >= 0) {
+            if (i0 + u < n) {
+                if (prog[i0 + u] >= 0) {
                     stack[sp * kWave + lane] = v[u];
                     ++sp;
                 } else {
@@ -818,7 +825,35 @@ __global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0
             }
         }
     }
-    const cd v = stack[lane];
+    return stack[lane];
+}
+
+// One part of the combine program: parts[p] = {first step, steps, slot of the sum}.
+// Workgroup = (part, block of 64 hafnians), placed by xcd_work.
+__global__ __launch_bounds__(kWave) void haf_combine_part_kernel(int64_t n_h, int64_t stride, double* vals,
+                                                                 const int* __restrict__ comb,
+                                                                 const int4* __restrict__ parts, int n_parts,
+                                                                 int n_hblocks) {
+    __shared__ cd stack[kCombineStack * kWave];
+    const int lane = threadIdx.x;
+    int pi, hb;
+    if (!xcd_work(n_parts, n_hblocks, pi, hb)) return;
+    const int64_t h = (int64_t)hb * kWave + lane;
+    if (h >= n_h) return;
+    const int4 part = parts[pi];
+    const cd v = combine_eval(vals, stride, h, comb + part.x, part.y, stack, lane);
+    st_cd(vals, (int64_t)part.z * stride + h, v);
+}
+
+template <class S>
+__global__ __launch_bounds__(kWave) void haf_combine_kernel(S src, int64_t item0, int64_t n_h, int64_t stride,
+                                                            const double* __restrict__ vals,
+                                                            const int* __restrict__ comb, int n_comb) {
+    __shared__ cd stack[kCombineStack * kWave];
+    const int lane = threadIdx.x;
+    const int64_t h = (int64_t)blockIdx.x * kWave + lane;
+    if (h >= n_h) return;
+    const cd v = combine_eval(vals, stride, h, comb, n_comb, stack, lane);
     const int64_t item = item0 + h;
     if constexpr (std::is_same<S, GammaSource>::value) src.store_dec(src.decode(item), v);
     else src.store(item, v);
@@ -1178,6 +1213,52 @@ struct FrontierPlan {
     std::vector<int> type_first, type_count;
     int* d_comb;
     int n_comb;
+    int4* d_parts;                   // combine parts {first step, steps, sum slot}
+    int n_parts;
+    int* d_top;                      // combine program above the parts (d_comb when none)
+    int n_top;
+};
+
+// Parts of a combine program: complete subtrees of its add tree of at most kCombinePartSteps
+// steps, found by descending from the root; "top" is the program of the tree above them, which
+// pushes each part's sum from the slot of the part's first value (a subtree's program starts
+// with its leftmost value).  Programs of at most 2 kCombinePartSteps steps are not cut.
+constexpr int kCombinePartSteps = 48;
+struct CombineCut {
+    struct Node {
+        int start, end, left, right;  // program range [start, end); children of an add
+    };
+    const std::vector<int>& prog;
+    std::vector<Node> nodes;
+    std::vector<int4> parts;
+    std::vector<int> top;
+    explicit CombineCut(const std::vector<int>& c) : prog(c) {
+        std::vector<int> st;
+        for (int i = 0; i < (int)prog.size(); ++i) {
+            if (prog[i] >= 0) {
+                nodes.push_back({i, i + 1, -1, -1});
+            } else {
+                const int r = st.back();
+                st.pop_back();
+                const int l = st.back();
+                st.pop_back();
+                nodes.push_back({nodes[l].start, i + 1, l, r});
+            }
+            st.push_back((int)nodes.size() - 1);
+        }
+        if (prog.size() > 2 * (size_t)kCombinePartSteps && st.size() == 1) cut(st.back());
+    }
+    void cut(int x) {
+        const Node c = nodes[x];
+        if (c.end - c.start <= kCombinePartSteps || c.left < 0) {
+            if (c.end - c.start > 1) parts.push_back(make_int4(c.start, c.end - c.start, prog[c.start], 0));
+            top.push_back(prog[c.start]);
+            return;
+        }
+        cut(c.left);
+        cut(c.right);
+        top.push_back(-1);
+    }
 };
 
 std::map<std::pair<int, int>, FrontierPlan> g_fplans;  // (device, m)
@@ -1240,6 +1321,17 @@ int get_fplan(int device, int m, FrontierPlan** out) {
             tlist.insert(tlist.end(), kv.second.begin(), kv.second.end());
         }
         p.n_comb = (int)fb.comb.size();
+        CombineCut cc(fb.comb);
+        p.n_parts = (int)cc.parts.size();
+        p.n_top = p.n_parts ? (int)cc.top.size() : p.n_comb;
+        for (const std::vector<int>* prog : {&fb.comb, &cc.top}) {
+            int dep = 0, mx = 0;
+            for (int c : *prog) {
+                dep += c >= 0 ? 1 : -1;
+                mx = std::max(mx, dep);
+            }
+            if (!prog->empty() && (mx > kCombineStack || dep != 1)) p.ok = false;
+        }
         int depth = 0, max_depth = 0;
         for (int c : fb.comb) {
             depth += c >= 0 ? 1 : -1;
@@ -1257,6 +1349,9 @@ int get_fplan(int device, int m, FrontierPlan** out) {
         BSMPS_TRY(upload(&p.d_front, fb.front));
         BSMPS_TRY(upload(&p.d_tlist, tlist));
         BSMPS_TRY(upload(&p.d_comb, fb.comb));
+        BSMPS_TRY(upload(&p.d_parts, cc.parts));
+        if (p.n_parts) BSMPS_TRY(upload(&p.d_top, cc.top));
+        else p.d_top = p.d_comb;
         it = g_fplans.emplace(key, p).first;
     }
     *out = &it->second;
@@ -1319,6 +1414,7 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
     int max_items = 1;
     for (int c : fp->level_count) max_items = std::max(max_items, c);
     for (int c : fp->type_count) max_items = std::max(max_items, c);
+    max_items = std::max(max_items, fp->n_parts);
     const int64_t max_hblocks = (((int64_t)1 << 32) / kWave - kXcds) / max_items;
     if (max_hblocks < 1) {
         bsmps_set_error("frontier-split hafnian of size %d: %d work items per launch exceed the grid limit", 2 * m,
@@ -1365,8 +1461,14 @@ int launch_split(bsmps_ctx* ctx, const S& src, int m, FrontierPlan* fp) {
                 bsmps_set_error("frontier-split hafnian path supports sizes 12..%d, got %d", BSMPS_MAX_HAFNIAN_SIZE, 2 * m);
                 return BSMPS_ERR_INVALID;
         }
+        if (fp->n_parts) {
+            hipLaunchKernelGGL(haf_combine_part_kernel, dim3((unsigned)xcd_grid(fp->n_parts, (int)hblocks)),
+                               dim3(kWave), 0, ctx->stream, nh, H, vals, fp->d_comb, fp->d_parts, fp->n_parts,
+                               (int)hblocks);
+            BSMPS_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL((haf_combine_kernel<S>), dim3(hblocks), dim3(kWave), 0, ctx->stream, src, c0, nh, H, vals,
-                           fp->d_comb, fp->n_comb);
+                           fp->d_top, fp->n_top);
         BSMPS_HIP(hipGetLastError());
     }
     return BSMPS_OK;
