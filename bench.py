@@ -643,8 +643,8 @@ def main():
         if not args.no_learned:
             from bosonsupremacy_amd import learned
             model = learned.load_or_train()
-        out = batched.run_batch(insts, n_samples=args.samples or 10 ** 4, verbose=True, learned_model=model)
-        out.update({"metric": "instances/s (construction + one graph-replayed sampling pass, FP64 or FP32 form)",
+        out = batched.run_batch_merged(insts, n_samples=args.samples or 10 ** 4, verbose=True, learned_model=model)
+        out.update({"metric": "instances/s (merged construction + one graph-replayed sampling pass per form)",
                     "workload": f"{args.batched} random instances, M~U{{16..96}}, d~U{{4..8}}, chi~U{{128..1024}}",
                     "tuning": {str(k): v for k, v in tuning.items()}})
         print(json.dumps(out))

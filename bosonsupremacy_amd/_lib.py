@@ -64,6 +64,7 @@ _PROTOTYPES = {
     "bsmps_loop_hafnian_targets": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p,
                                     c_void_p], c_int),
     "bsmps_gamma_site": ([c_void_p, ctypes.POINTER(GammaSiteArgs), c_void_p], c_int),
+    "bsmps_gamma_sites": ([c_void_p, c_int, ctypes.POINTER(GammaSiteArgs), ctypes.POINTER(c_void_p)], c_int),
     "bsmps_kron_cuts": ([c_void_p, c_int, c_int32_p, c_int, c_int, c_void_p, c_int64, c_int, c_double,
                          c_void_p, c_void_p, c_void_p], c_int),
     "bsmps_kron_cuts_noalloc": ([c_void_p, c_int, c_int32_p, c_int, c_int, c_void_p, c_int64, c_int, c_double,

@@ -210,3 +210,199 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
     out["graph_capture_s"] = {f: v[1] for f, v in cap.items()}
     out["per_instance"] = rows
     return out
+
+
+def _pass_and_errors(inst, G, L, sqrtW, names, n_samples, now, plan=None, learned_precision="fp16x3"):
+    """One instance's sampling in every form of ``names`` (the first is FP64): a graph per form,
+    one replay with the Philox uniforms (timed), and each reduced form replayed along the FP64
+    sample paths (mid-interval uniforms).  ``plan`` = (site_d, site_chi) adds the learned-cutoff
+    form: the leading blocks of the pruned tensors that the plan keeps, in ``learned_precision``.
+    Returns (times, row, errors per form)."""
+    from . import metrics
+    times, row, errs = {}, {"M": inst.M, "d": inst.d, "chi": inst.chi}, {}
+    forms = list(names)
+    shapes = {f: (G, L) for f in forms}
+    if plan is not None:
+        site_d, site_chi = plan
+        M = inst.M
+        Gl, Ll = [], []
+        for k in range(M):
+            rows = site_chi[k - 1] if k > 0 else G[k].shape[0]
+            cols = site_chi[k] if k < M - 1 else G[k].shape[1]
+            Gl.append(G[k][:rows, :cols, :site_d[k]].contiguous())
+            Ll.append(None if L[k] is None else np.asarray(L[k])[:cols])
+        forms.append("learned")
+        shapes["learned"] = (Gl, Ll)
+    graphs = {}
+    for f in forms:
+        ta = now()
+        g, lam = shapes[f]
+        graphs[f] = GraphSampler(g, lam, sqrtW, inst.dd, n_samples, learned_precision if f == "learned" else f,
+                                 return_probs=True)
+        times["setup_" + f] = now() - ta
+    normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=inst.seed)
+    ta = now()
+    s64, p64 = graphs[forms[0]].run(normals, uniforms)
+    times["sampling_" + forms[0]] = now() - ta
+    mid = mid_uniforms(s64, p64)
+    e1_64, e2_64 = metrics.rb_first_order(p64), metrics.rb_second_order(s64, p64)
+    big = np.abs(e2_64) > 1e-3 * np.abs(e2_64).max()
+    m64 = p64.mean(dim=1)
+    for f in forms[1:]:
+        ta = now()
+        sf, _ = graphs[f].run(normals, uniforms)
+        times["sampling_" + f] = now() - ta
+        flips = int((sf != s64).any(dim=1).sum())
+        sf, pf = graphs[f].run(normals, mid)
+        e1, e2 = metrics.rb_first_order(pf), metrics.rb_second_order(sf, pf)
+        r1 = float(np.max(np.abs(e1 - e1_64) / np.abs(e1_64)))
+        r2 = float(np.max(np.abs(e2[big] - e2_64[big]) / np.abs(e2_64[big]))) if big.any() else 0.0
+        err = float(torch.abs(pf.mean(dim=1) - m64).max())
+        errs[f] = (err, r1, r2)
+        row.update({f"{f}_marginal_abs_err": err, f"{f}_first_order_rel_err": r1, f"{f}_second_order_rel_err": r2,
+                    f"{f}_draws_differing_same_uniforms": flips,
+                    f"{f}_off_fp64_path": int((sf != s64).any(dim=1).sum())})
+    return times, row, errs
+
+
+def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), group_bytes=32 << 30,
+                     verbose=False, learned_model=None, learned_precision="fp16x3"):
+    """Batched form with cross-instance construction.  Instances are taken in groups whose site
+    tensors together fit ``group_bytes``; the site tensors of a whole group -- every site of
+    every instance in it -- are built by one ``bsmps_gamma_sites`` call, i.e. one hafnian launch
+    per photon total over the (site, j, p, q) rectangles of all of them, whatever their cutoffs,
+    ranks and hafnian sizes.  While the GPU builds group g the host prepares group g + 1: its
+    Schmidt spectra on a side stream with a context of their own, its FP64 site kernels and
+    label tables in the worker pool, its uploads and output tensors.  Each instance of group g
+    is then sampled as in run_batch: one HIP graph per form (the pruned plain form's tensors,
+    FP64 first), replayed once with the Philox uniforms and once along the FP64 sample paths,
+    with relative errors of the Rao-Blackwellised marginals.  With ``learned_model`` each
+    instance's exactly judged plan is walked as one more form on the leading blocks of its
+    pruned tensors (the tensors that plan would build), so its construction is the pruned one.
+    Instances whose hafnians exceed the library's sizes are skipped and listed.
+
+    Timing: the sampling of each instance (graphs, passes, error replays) is timed as a block;
+    construction is the rest of the wall time.  instances/s of a form = instances /
+    (construction + that form's graph set-up and pass)."""
+    from . import kron as kron_mod, hostpool, learned as learned_mod
+
+    def now():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+    t_start = now()
+    names = ("fp64",) + tuple(forms)
+    all_forms = names + (("learned",) if learned_model is not None else ())
+    tot = {f: 0.0 for f in ("check", "prepare", "merged_launch")}
+    for f in all_forms:
+        tot["setup_" + f] = tot["sampling_" + f] = 0.0
+    errs = {f: [] for f in all_forms[1:]}
+    rows, skipped, group_sizes = [], [], []
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        kctx = runtime.Context(runtime.context().device.index)
+    max_total = _lib_max_hafnian_size() + 1   # odd photon totals are never computed
+
+    def prepare(i, inst):
+        """Spectra (side stream), host work and uploads of one instance: (i, inst, calls, kron)
+        or None when it is skipped."""
+        cuts = hostpool.cut_inputs_all(inst.sq_cov, inst.d)
+        with torch.cuda.stream(side):
+            spec = kron_mod.kron_cuts([tp for tp, _ in cuts], inst.chi, ctx=kctx)
+        kron = ([r for r, _ in spec], [n for _, n in spec], [S for _, S in cuts])
+        lc = [int(np.asarray(n).reshape(len(np.asarray(r)), -1)[np.asarray(r) > mps.ERR_TOL].sum(axis=1).max(initial=0))
+              for r, n in zip(kron[0], kron[1])]
+        # the largest photon total of any site: d - 1 + (photons of a label of cut k) + (of cut k - 1)
+        top = max([inst.d - 1 + (lc[k] if k < inst.M - 1 else 0) + (lc[k - 1] if k > 0 else 0)
+                   for k in range(inst.M)])
+        if top > max_total:
+            skipped.append({"index": i, "M": inst.M, "d": inst.d, "chi": inst.chi,
+                            "reason": f"hafnians up to size {top}"})
+            return None
+        calls, _ = mps.site_calls(inst.sq_cov, inst.d, inst.chi, kron=kron, pruned=True)
+        return (i, inst, calls, kron)
+
+    work = iter(enumerate(instances))
+    carry = [None]
+
+    def next_group():
+        """The next group, prepared on the host (nothing waits for the main stream)."""
+        t0 = time.perf_counter()
+        group, nbytes = [], 0
+        if carry[0] is not None:
+            group.append(carry[0])
+            nbytes = sum(c.gamma.numel() * 16 for c in carry[0][2])
+            carry[0] = None
+        for i, inst in work:
+            item = prepare(i, inst)
+            if item is None:
+                continue
+            b = sum(c.gamma.numel() * 16 for c in item[2])
+            if group and nbytes + b > group_bytes:
+                carry[0] = item
+                break
+            group.append(item)
+            nbytes += b
+        tot["prepare"] += time.perf_counter() - t0
+        return group
+
+    def launch(group):
+        t0 = time.perf_counter()
+        mps.gamma_sites([c for _, _, calls, _ in group for c in calls])
+        tot["merged_launch"] += time.perf_counter() - t0
+        group_sizes.append(len(group))
+
+    def check(group):
+        for i, inst, calls, kron in group:
+            t0 = now()
+            G = [c.gamma for c in calls]
+            L = [c.Lambda for c in calls]
+            plan = None
+            if learned_model is not None:
+                site_d, site_chi, _ = learned_mod.plan_sites(learned_model, inst.sq_cov, inst.cov, kron[0],
+                                                             inst.d, inst.chi)
+                plan = (site_d, site_chi)
+            times, row, e = _pass_and_errors(inst, G, L, sampling.thermal_sqrtW(inst.cov, inst.sq_cov), names,
+                                             n_samples, now, plan, learned_precision)
+            for k, v in times.items():
+                tot[k] += v
+            for f, v in e.items():
+                errs[f].append(v)
+            row["index"] = i
+            rows.append(row)
+            tot["check"] += now() - t0
+            if verbose:
+                print(f"[batched] instance {i}: M={inst.M} d={inst.d} chi={inst.chi} "
+                      + " ".join(f"{f} {times['sampling_' + f]:.3f} s" for f in all_forms), file=sys.stderr, flush=True)
+
+    group = next_group()
+    while group:
+        launch(group)             # enqueued; the GPU builds this group ...
+        nxt = next_group()        # ... while the host prepares the next one
+        check(group)              # waits for the construction, then samples and checks
+        del group
+        group = nxt
+    wall = now() - t_start
+    done = len(rows)
+    construction = wall - tot["check"]
+    out = {"instances": done, "skipped": skipped, "samples_per_instance": n_samples, "wall_s": wall,
+           "construction": "merged: one bsmps_gamma_sites call per group of instances, the next group "
+                           "prepared on the host meanwhile",
+           "groups": len(group_sizes), "instances_per_group": group_sizes, "construction_s": construction,
+           "host_prepare_s": tot["prepare"], "sampling_and_checks_s": tot["check"]}
+    for f in all_forms:
+        out[f"instances_per_s_{f}"] = done / (construction + tot["setup_" + f] + tot["sampling_" + f])
+    out.update({k + "_s": v for k, v in tot.items() if k.startswith(("setup_", "sampling_"))})
+    for f in all_forms[1:]:
+        e = errs[f]
+        out[f"{f}_marginal_abs_err_max"] = float(np.max([x[0] for x in e])) if e else 0.0
+        out[f"{f}_first_order_rel_err_max"] = float(np.max([x[1] for x in e])) if e else 0.0
+        out[f"{f}_second_order_rel_err_max"] = float(np.max([x[2] for x in e])) if e else 0.0
+        out[f"{f}_off_fp64_path_total"] = int(sum(r.get(f"{f}_off_fp64_path", 0) for r in rows))
+    out["graph_setup_s"] = {f: tot["setup_" + f] for f in all_forms}
+    out["per_instance"] = rows
+    return out
+
+
+def _lib_max_hafnian_size():
+    from . import _lib
+    return _lib.MAX_HAFNIAN_SIZE

@@ -39,6 +39,12 @@ namespace {
 constexpr int kWave = 64;
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int tri(int J, int K) { return J * (J - 1) / 2 + K; }
+// Row J of packed index i = tri(J, K), 0 <= K < J (folds to a constant for a constant i).
+__host__ __device__ constexpr int tri_row(int i) {
+    int J = 1;
+    while ((J + 1) * J / 2 <= i) ++J;
+    return J;
+}
 
 // Highest polynomial coefficient of matrix entry (J, K), J > K, that can still reach the
 // leaf value g[m] of a 2m x 2m hafnian.  A coefficient of degree t reaches g only by raising
@@ -95,70 +101,83 @@ struct DenseSource {  // hafnian of explicit matrices
     __device__ void store(int64_t item, cd h) const { st_cd(out, item, h); }
 };
 
-struct GammaSource {  // site tensor entries Gamma[b, a, j] of one photon total n
+// Tables of one site tensor, in device memory: Sigma, the per-label targets, denominators and
+// photon-class permutations of both sides, Lambda, and where Gamma goes.
+struct GammaSite {
     const double* sigma;
-    int ld;
-    int d, chi;
-    int vacuum_real_div;
-    double z, inv_z;
     const double* sqrt_fact;
     const int32_t* lt;
-    int wl;
     const double* lden;
     const double* lam;
     const int32_t* lperm;
     const int32_t* loff;  // class offsets into lperm
     const int32_t* rt;
-    int wr;
     const double* rden;
     const int32_t* rperm;
     const int32_t* roff;
     const int32_t* rcnt;  // class counts (right)
-    const int4* rects;    // {j, p, q, start} for this launch, sorted by start
-    int n_rects;
     double* gamma;
+    double z, inv_z;
+    int ld, d, chi, wl, wr, vacuum_real_div;
+};
+
+// A rectangle class_L[p] x class_R[q] x {j} of one site's (j, a, b) items of one photon total;
+// items are numbered consecutively over the rectangles of a launch (start = first item).
+struct GammaRect {
+    int64_t start;
+    int j, p, q, site;
+};
+
+struct GammaSource {  // site tensor entries Gamma[b, a, j] of one photon total n, of one or many sites
+    const GammaSite* sites;
+    const GammaRect* rects;  // sorted by start
+    int n_rects;
     int64_t n_items;
 
     struct Decoded {
         int j, p, q, a, b;
+        const GammaSite* s;
     };
     __device__ Decoded decode(int64_t item) const {
         int lo = 0, hi = n_rects - 1;
         while (lo < hi) {  // last rect with start <= item
             int mid = (lo + hi + 1) >> 1;
-            if (rects[mid].w <= item) lo = mid;
+            if (rects[mid].start <= item) lo = mid;
             else hi = mid - 1;
         }
-        const int4 r = rects[lo];
-        const int64_t off = item - r.w;
-        const int cr = rcnt[r.z];
+        const GammaRect r = rects[lo];
+        const GammaSite* st = sites + r.site;
+        const int64_t off = item - r.start;
+        const int cr = st->rcnt[r.q];
         const int64_t ia = off / cr;
         const int ib = (int)(off - ia * cr);
         Decoded dc;
-        dc.j = r.x;
-        dc.p = r.y;
-        dc.q = r.z;
-        dc.a = lperm[loff[r.y] + ia];
-        dc.b = rperm[roff[r.z] + ib];
+        dc.j = r.j;
+        dc.p = r.p;
+        dc.q = r.q;
+        dc.a = st->lperm[st->loff[r.p] + ia];
+        dc.b = st->rperm[st->roff[r.q] + ib];
+        dc.s = st;
         return dc;
     }
     __device__ void fetch_dec(const Decoded& dc, int k, int& tk) const {
         if (k < dc.j) tk = 0;
-        else if (k < dc.j + dc.p) tk = lt[(int64_t)dc.a * wl + (k - dc.j)];
-        else tk = rt[(int64_t)dc.b * wr + (k - dc.j - dc.p)];
+        else if (k < dc.j + dc.p) tk = dc.s->lt[(int64_t)dc.a * dc.s->wl + (k - dc.j)];
+        else tk = dc.s->rt[(int64_t)dc.b * dc.s->wr + (k - dc.j - dc.p)];
     }
-    __device__ const double* base(int64_t) const { return sigma; }
-    __device__ int ldim() const { return ld; }
+    __device__ static const double* base_dec(const Decoded& dc) { return dc.s->sigma; }
+    __device__ static int ldim_dec(const Decoded& dc) { return dc.s->ld; }
     __device__ void store_dec(const Decoded& dc, cd h) const {
+        const GammaSite* st = dc.s;
         cd v;
-        if (vacuum_real_div && dc.j + dc.p + dc.q == 0) {
-            v.re = (1.0 / z) / lam[dc.a];
+        if (st->vacuum_real_div && dc.j + dc.p + dc.q == 0) {
+            v.re = (1.0 / st->z) / st->lam[dc.a];
             v.im = 0.0;
         } else {
-            const double den = (sqrt_fact[dc.j] * lden[dc.a]) * rden[dc.b];
-            v = cscale(cscale(cscale(h, 1.0 / den), inv_z), 1.0 / lam[dc.a]);
+            const double den = (st->sqrt_fact[dc.j] * st->lden[dc.a]) * st->rden[dc.b];
+            v = cscale(cscale(cscale(h, 1.0 / den), st->inv_z), 1.0 / st->lam[dc.a]);
         }
-        st_cd(gamma, ((int64_t)dc.b * chi + dc.a) * d + dc.j, v);
+        st_cd(st->gamma, ((int64_t)dc.b * st->chi + dc.a) * st->d + dc.j, v);
     }
 };
 
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(kWave) void haf_lane_kernel(S src, int64_t item0) {
         const auto dc = src.decode(item);
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) src.fetch_dec(dc, k, t[k]);
-        const cd h = lane_hafnian<M>(t, src.base(item), src.ldim(), lds + lane, kWave);
+        const cd h = lane_hafnian<M>(t, GammaSource::base_dec(dc), GammaSource::ldim_dec(dc), lds + lane, kWave);
         src.store_dec(dc, h);
     } else {
 #pragma unroll
@@ -500,8 +519,15 @@ __global__ __launch_bounds__(kWave) void haf_group_kernel(S src, int64_t item0, 
         tg[k] = tk;
     }
     lds_barrier();
-    const double* Sg = src.base(eff);
-    const int ld = src.ldim();
+    const double* Sg;
+    int ld;
+    if constexpr (std::is_same<S, GammaSource>::value) {
+        Sg = GammaSource::base_dec(dc);
+        ld = GammaSource::ldim_dec(dc);
+    } else {
+        Sg = src.base(eff);
+        ld = src.ldim();
+    }
     for (int i = gl; i < m * (2 * m - 1); i += G) {
         int J, K;
         decode_pair(i, J, K);
@@ -642,21 +668,39 @@ __global__ __launch_bounds__(kWave) void haf_bfs_prep_kernel(S src, int64_t item
     if (h >= bd.n_h) return;
     const int64_t item = item0 + h;
     int t[2 * M];
+    const double* Sg;
+    int ld;
     if constexpr (std::is_same<S, GammaSource>::value) {
         const auto dc = src.decode(item);
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) src.fetch_dec(dc, k, t[k]);
+        Sg = GammaSource::base_dec(dc);
+        ld = GammaSource::ldim_dec(dc);
     } else {
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) src.fetch(item, k, t[k]);
+        Sg = src.base(item);
+        ld = src.ldim();
     }
-    const double* Sg = src.base(item);
-    const int ld = src.ldim();
+    // the gathered entries go out in batches of 8, each batch loaded before any of it is stored
+    // (a load placed after a store to possibly the same memory waits for nothing but cannot be
+    // moved above it, so one-at-a-time gathers would each wait a full latency)
+    constexpr int NA = M * (2 * M - 1);
+    constexpr int kB = 8;
 #pragma unroll
-    for (int J = 1; J < 2 * M; ++J)
+    for (int i0 = 0; i0 < NA; i0 += kB) {
+        cd v[kB];
 #pragma unroll
-        for (int K = 0; K < J; ++K)
-            st_cd(bd.data, (int64_t)(bd.a_off + tri(J, K)) * bd.stride + h, ld_cd(Sg, (int64_t)t[J] * ld + t[K]));
+        for (int u = 0; u < kB; ++u) {
+            if (i0 + u < NA) {
+                const int J = tri_row(i0 + u), K = i0 + u - tri(J, 0);
+                v[u] = ld_cd(Sg, (int64_t)t[J] * ld + t[K]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+            if (i0 + u < NA) st_cd(bd.data, (int64_t)(bd.a_off + i0 + u) * bd.stride + h, v[u]);
+    }
 #pragma unroll
     for (int i = 0; i <= M; ++i)
         st_cd(bd.data, (int64_t)(rootg + i) * bd.stride + h, i == 0 ? cd{1.0, 0.0} : czero());
@@ -1654,132 +1698,187 @@ extern "C" int bsmps_loop_hafnian_targets(bsmps_ctx* ctx, const double* sigma, i
     return launch_loop(ctx, s, loops, 0, n_select);
 }
 
-extern "C" int bsmps_gamma_site(bsmps_ctx* ctx, const bsmps_gamma_site_args* a, double* gamma) {
-    if (!ctx || !a || !gamma) {
-        bsmps_set_error("bsmps_gamma_site: null argument");
+namespace {
+
+// Checks one site's arguments and derives its class offsets and counts.
+int gamma_site_tables(const bsmps_gamma_site_args* a, const char* who, std::vector<int32_t>& loff,
+                      std::vector<int32_t>& roff, std::vector<int32_t>& rcnt, int& max_n) {
+    if (!a) {
+        bsmps_set_error("%s: null argument", who);
         return BSMPS_ERR_INVALID;
     }
     if (a->kind < 0 || a->kind > 2 || a->d <= 0 || a->chi <= 0 || a->ld <= 0 || a->n_left < 0 || a->n_right < 0) {
-        bsmps_set_error("bsmps_gamma_site: invalid sizes (kind %d, d %d, chi %d, ld %d, n_left %d, n_right %d)",
+        bsmps_set_error("%s: invalid sizes (kind %d, d %d, chi %d, ld %d, n_left %d, n_right %d)", who,
                         a->kind, a->d, a->chi, a->ld, a->n_left, a->n_right);
         return BSMPS_ERR_INVALID;
     }
     if (a->n_left > a->chi) {
-        bsmps_set_error("bsmps_gamma_site: %d left labels exceed chi = %d", a->n_left, a->chi);
+        bsmps_set_error("%s: %d left labels exceed chi = %d", who, a->n_left, a->chi);
         return BSMPS_ERR_INVALID;
     }
     if (a->n_left_class <= 0 || a->n_right_class <= 0) {
-        bsmps_set_error("bsmps_gamma_site: empty photon classes (%d left, %d right)", a->n_left_class,
-                        a->n_right_class);
+        bsmps_set_error("%s: empty photon classes (%d left, %d right)", who, a->n_left_class, a->n_right_class);
         return BSMPS_ERR_INVALID;
     }
     if (!a->sigma || !a->sqrt_fact || !a->left_den || !a->lambda || !a->left_perm || !a->right_den ||
         !a->right_perm || !a->left_class_count_host || !a->right_class_count_host) {
-        bsmps_set_error("bsmps_gamma_site: null pointer (sigma %p sqrt_fact %p left_den %p lambda %p left_perm %p "
-                        "right_den %p right_perm %p)", (const void*)a->sigma, (const void*)a->sqrt_fact,
+        bsmps_set_error("%s: null pointer (sigma %p sqrt_fact %p left_den %p lambda %p left_perm %p "
+                        "right_den %p right_perm %p)", who, (const void*)a->sigma, (const void*)a->sqrt_fact,
                         (const void*)a->left_den, (const void*)a->lambda, (const void*)a->left_perm,
                         (const void*)a->right_den, (const void*)a->right_perm);
         return BSMPS_ERR_INVALID;
     }
     const int nlc = a->n_left_class, nrc = a->n_right_class;
     if (nlc - 1 > a->w_left || nrc - 1 > a->w_right) {
-        bsmps_set_error("bsmps_gamma_site: photon classes exceed target widths");
+        bsmps_set_error("%s: photon classes exceed target widths", who);
         return BSMPS_ERR_INVALID;
     }
-    // class offsets and counts
-    std::vector<int32_t> loff(nlc + 1, 0), roff(nrc + 1, 0), rcnt(nrc, 0);
+    loff.assign(nlc + 1, 0);
+    roff.assign(nrc + 1, 0);
+    rcnt.assign(nrc, 0);
     int64_t lsum = 0, rsum = 0;
     for (int p = 0; p < nlc; ++p) {
-        if (a->left_class_count_host[p] < 0) { bsmps_set_error("bsmps_gamma_site: negative class count"); return BSMPS_ERR_INVALID; }
+        if (a->left_class_count_host[p] < 0) {
+            bsmps_set_error("%s: negative class count", who);
+            return BSMPS_ERR_INVALID;
+        }
         loff[p + 1] = loff[p] + a->left_class_count_host[p];
         lsum += a->left_class_count_host[p];
     }
     for (int q = 0; q < nrc; ++q) {
-        if (a->right_class_count_host[q] < 0) { bsmps_set_error("bsmps_gamma_site: negative class count"); return BSMPS_ERR_INVALID; }
+        if (a->right_class_count_host[q] < 0) {
+            bsmps_set_error("%s: negative class count", who);
+            return BSMPS_ERR_INVALID;
+        }
         roff[q + 1] = roff[q] + a->right_class_count_host[q];
         rcnt[q] = a->right_class_count_host[q];
         rsum += a->right_class_count_host[q];
     }
     if (lsum != a->n_left || rsum != a->n_right) {
-        bsmps_set_error("bsmps_gamma_site: class counts do not add up to the label counts");
+        bsmps_set_error("%s: class counts do not add up to the label counts", who);
         return BSMPS_ERR_INVALID;
     }
-    const int max_n = (a->d - 1) + (nlc - 1) + (nrc - 1);
+    max_n = (a->d - 1) + (nlc - 1) + (nrc - 1);
     if (max_n > BSMPS_MAX_HAFNIAN_SIZE + 1) {
-        bsmps_set_error("bsmps_gamma_site: hafnians up to size %d exceed the supported %d", max_n,
-                        BSMPS_MAX_HAFNIAN_SIZE);
+        bsmps_set_error("%s: hafnians up to size %d exceed the supported %d", who, max_n, BSMPS_MAX_HAFNIAN_SIZE);
         return BSMPS_ERR_UNSUPPORTED;
     }
-    // rectangles (j, p, q) grouped by even total n = j + p + q, items numbered per n
-    std::vector<int4> rects;
-    std::vector<int> n_first(max_n + 2, 0);
+    return BSMPS_OK;
+}
+
+// Site tensors of any number of sites: for each even photon total n (largest first) one hafnian
+// launch covers the (j, p, q) rectangles of every site, numbered site after site.  The tables go
+// to the context workspace as [GammaSite x n_sites][GammaRect...][int32 class tables].
+int gamma_sites(bsmps_ctx* ctx, int n_sites, const bsmps_gamma_site_args* args, double* const* gammas,
+                const char* who) {
+    if (!ctx || n_sites < 0 || (n_sites > 0 && (!args || !gammas))) {
+        bsmps_set_error("%s: null argument", who);
+        return BSMPS_ERR_INVALID;
+    }
+    std::vector<std::vector<int32_t>> loff(n_sites), roff(n_sites), rcnt(n_sites);
+    int max_n = -1;
+    for (int i = 0; i < n_sites; ++i) {
+        int mn = 0;
+        BSMPS_TRY(gamma_site_tables(&args[i], who, loff[i], roff[i], rcnt[i], mn));
+        if (!gammas[i]) {
+            bsmps_set_error("%s: null output of site %d", who, i);
+            return BSMPS_ERR_INVALID;
+        }
+        max_n = std::max(max_n, mn);
+    }
+    if (max_n < 0) return BSMPS_OK;
+    // rectangles grouped by even total n, each group numbered site after site
+    std::vector<std::vector<GammaRect>> by_n(max_n + 1);
     std::vector<int64_t> n_items(max_n + 1, 0);
-    for (int n = 0; n <= max_n; ++n) {
-        n_first[n] = (int)rects.size();
-        if (n % 2) continue;
+    for (int n = 0; n <= max_n; n += 2) {
         int64_t start = 0;
-        for (int j = 0; j < a->d && j <= n; ++j)
-            for (int p = 0; p < nlc && j + p <= n; ++p) {
-                const int q = n - j - p;
-                if (q >= nrc) continue;
-                const int64_t cnt = (int64_t)a->left_class_count_host[p] * a->right_class_count_host[q];
-                if (cnt == 0) continue;
-                if (start > INT32_MAX) { bsmps_set_error("bsmps_gamma_site: too many entries of one size"); return BSMPS_ERR_UNSUPPORTED; }
-                rects.push_back(make_int4(j, p, q, (int)start));
-                start += cnt;
-            }
+        for (int i = 0; i < n_sites; ++i) {
+            const bsmps_gamma_site_args* a = &args[i];
+            for (int j = 0; j < a->d && j <= n; ++j)
+                for (int p = 0; p < a->n_left_class && j + p <= n; ++p) {
+                    const int q = n - j - p;
+                    if (q >= a->n_right_class) continue;
+                    const int64_t cnt = (int64_t)a->left_class_count_host[p] * a->right_class_count_host[q];
+                    if (cnt == 0) continue;
+                    by_n[n].push_back(GammaRect{start, j, p, q, i});
+                    start += cnt;
+                }
+        }
         n_items[n] = start;
     }
-    n_first[max_n + 1] = (int)rects.size();
-    // upload tables into the context workspace: rects | loff | roff | rcnt
-    const size_t bytes_rects = sizeof(int4) * std::max<size_t>(rects.size(), 1);
-    const size_t bytes_tab = sizeof(int32_t) * (loff.size() + roff.size() + rcnt.size());
+    size_t n_rects = 0, n_tab = 0;
+    for (const auto& v : by_n) n_rects += v.size();
+    for (int i = 0; i < n_sites; ++i) n_tab += loff[i].size() + roff[i].size() + rcnt[i].size();
+    const size_t bytes_sites = sizeof(GammaSite) * (size_t)n_sites;
+    const size_t bytes_rects = sizeof(GammaRect) * std::max<size_t>(n_rects, 1);
+    const size_t bytes_tab = sizeof(int32_t) * n_tab;
     void* ws = nullptr;
-    BSMPS_TRY(bsmps_ctx_workspace(ctx, bytes_rects + bytes_tab + 256, &ws));
-    std::vector<char> host(bytes_rects + bytes_tab);
-    memcpy(host.data(), rects.data(), sizeof(int4) * rects.size());
-    char* p = host.data() + bytes_rects;
-    memcpy(p, loff.data(), sizeof(int32_t) * loff.size());
-    p += sizeof(int32_t) * loff.size();
-    memcpy(p, roff.data(), sizeof(int32_t) * roff.size());
-    p += sizeof(int32_t) * roff.size();
-    memcpy(p, rcnt.data(), sizeof(int32_t) * rcnt.size());
+    BSMPS_TRY(bsmps_ctx_workspace(ctx, bytes_sites + bytes_rects + bytes_tab + 256, &ws));
+    char* dbase = reinterpret_cast<char*>(ws);
+    std::vector<char> host(bytes_sites + bytes_rects + bytes_tab);
+    GammaSite* hs = reinterpret_cast<GammaSite*>(host.data());
+    int32_t* htab = reinterpret_cast<int32_t*>(host.data() + bytes_sites + bytes_rects);
+    const int32_t* dtab = reinterpret_cast<const int32_t*>(dbase + bytes_sites + bytes_rects);
+    size_t t = 0;
+    for (int i = 0; i < n_sites; ++i) {
+        const bsmps_gamma_site_args* a = &args[i];
+        GammaSite& g = hs[i];
+        g.sigma = a->sigma;
+        g.sqrt_fact = a->sqrt_fact;
+        g.lt = a->left_targets;
+        g.lden = a->left_den;
+        g.lam = a->lambda;
+        g.lperm = a->left_perm;
+        g.rt = a->right_targets;
+        g.rden = a->right_den;
+        g.rperm = a->right_perm;
+        g.gamma = gammas[i];
+        g.z = a->z;
+        g.inv_z = a->inv_z;
+        g.ld = a->ld;
+        g.d = a->d;
+        g.chi = a->chi;
+        g.wl = a->w_left;
+        g.wr = a->w_right;
+        g.vacuum_real_div = a->kind != BSMPS_SITE_FIRST;
+        const std::vector<int32_t>* tabs[3] = {&loff[i], &roff[i], &rcnt[i]};
+        const int32_t** dst[3] = {&g.loff, &g.roff, &g.rcnt};
+        for (int k = 0; k < 3; ++k) {
+            *dst[k] = dtab + t;
+            memcpy(htab + t, tabs[k]->data(), sizeof(int32_t) * tabs[k]->size());
+            t += tabs[k]->size();
+        }
+    }
+    GammaRect* hr = reinterpret_cast<GammaRect*>(host.data() + bytes_sites);
+    const GammaRect* dr = reinterpret_cast<const GammaRect*>(dbase + bytes_sites);
+    std::vector<size_t> n_first(max_n + 1, 0);
+    size_t r = 0;
+    for (int n = 0; n <= max_n; ++n) {
+        n_first[n] = r;
+        for (const GammaRect& x : by_n[n]) hr[r++] = x;
+    }
     BSMPS_TRY(bsmps_ctx_upload(ctx, host.data(), host.size(), ws));
-    const int4* d_rects = reinterpret_cast<const int4*>(ws);
-    const int32_t* d_loff = reinterpret_cast<const int32_t*>(reinterpret_cast<char*>(ws) + bytes_rects);
-    const int32_t* d_roff = d_loff + loff.size();
-    const int32_t* d_rcnt = d_roff + roff.size();
-
-    GammaSource base{};
-    base.sigma = a->sigma;
-    base.ld = a->ld;
-    base.d = a->d;
-    base.chi = a->chi;
-    base.vacuum_real_div = a->kind != BSMPS_SITE_FIRST;
-    base.z = a->z;
-    base.inv_z = a->inv_z;
-    base.sqrt_fact = a->sqrt_fact;
-    base.lt = a->left_targets;
-    base.wl = a->w_left;
-    base.lden = a->left_den;
-    base.lam = a->lambda;
-    base.lperm = a->left_perm;
-    base.loff = d_loff;
-    base.rt = a->right_targets;
-    base.wr = a->w_right;
-    base.rden = a->right_den;
-    base.rperm = a->right_perm;
-    base.roff = d_roff;
-    base.rcnt = d_rcnt;
-    base.gamma = gamma;
+    const GammaSite* ds = reinterpret_cast<const GammaSite*>(dbase);
     // largest sizes first: they are the longest-running launches
     for (int n = max_n; n >= 0; --n) {
         if (n % 2 || n_items[n] == 0) continue;
-        GammaSource s = base;
-        s.rects = d_rects + n_first[n];
-        s.n_rects = n_first[n + 1] - n_first[n];
-        s.n_items = n_items[n];
-        BSMPS_TRY(launch_hafnians(ctx, s, n));
+        GammaSource src{ds, dr + n_first[n], (int)by_n[n].size(), n_items[n]};
+        BSMPS_TRY(launch_hafnians(ctx, src, n));
     }
     return BSMPS_OK;
+}
+
+}  // namespace
+
+extern "C" int bsmps_gamma_site(bsmps_ctx* ctx, const bsmps_gamma_site_args* a, double* gamma) {
+    if (!ctx || !a || !gamma) {
+        bsmps_set_error("bsmps_gamma_site: null argument");
+        return BSMPS_ERR_INVALID;
+    }
+    return gamma_sites(ctx, 1, a, &gamma, "bsmps_gamma_site");
+}
+
+extern "C" int bsmps_gamma_sites(bsmps_ctx* ctx, int n_sites, const bsmps_gamma_site_args* args,
+                                 double* const* gammas) {
+    return gamma_sites(ctx, n_sites, args, gammas, "bsmps_gamma_sites");
 }

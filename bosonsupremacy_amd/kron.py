@@ -34,15 +34,17 @@ def cut_spectrum_inputs(sq_cov, L, cutoff):
     return gaussian.cut_inputs(sq_cov, L, cutoff)
 
 
-def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None, no_alloc=False):
+def kron_cuts(tps, chi, max_dim=MAX_DIM, err_tol=ERR_TOL, device=0, timer=None, no_alloc=False, ctx=None):
     """Top-chi Schmidt weights and labels of several cuts at once on the GPU.
 
     tps: list of arrays [n_c, d] (thermal distributions of each cut's modes).
     Returns a list of (res float64 [len], num int8 [len, n_c] -- 1-D when n_c == 1).
     ``no_alloc`` runs the library's allocation-free entry point (fixed device scratch; the
-    same results).
+    same results).  ``ctx`` (default: the device's process-wide context) is the library
+    context to enqueue on; run under ``torch.cuda.stream(s)`` for a context bound to s, so the
+    buffers and the copy back are ordered on s.
     """
-    ctx = runtime.context(device)
+    ctx = ctx or runtime.context(device)
     n_cuts = len(tps)
     if n_cuts == 0:
         return []

@@ -235,6 +235,63 @@ def _i32(a):
     return a if a.size else np.zeros(1, np.int32)
 
 
+class SiteCall:
+    """One site tensor's library call: its ``bsmps_gamma_site_args``, the zeroed output Gamma
+    (device, [n_b][n_a][d]), Lambda_k, and the host and device tables the arguments point to
+    (kept alive with the call)."""
+
+    __slots__ = ("k", "args", "gamma", "Lambda", "tables", "uploads", "lcc", "rcc")
+
+
+def site_call(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, sigma_z=None,
+              ranks=None, tables=None):
+    """The SiteCall of site k (MPS_cpu.py:402-520): Sigma and Z of the site (host FP64), its label
+    tables uploaded in one pinned copy, and Gamma allocated on the device.  Arguments as for
+    site_tensor; nothing runs on the GPU yet."""
+    ctx = runtime.context()
+    dev = ctx.device
+    Sigma, Z = sigma_z if sigma_z is not None else gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
+    t = tables if tables is not None else site_tables(k, M, d, chi, kron_res, kron_num, err_tol, ranks)
+    Mp = Sigma.shape[0] // 2
+    if k > 0 and Mp != M - k:
+        raise ValueError(f"site {k}: Sigma has {Mp} modes, expected {M - k}")
+    left, right = t.left, t.right
+    # every table of the site in one pinned staging buffer and one asynchronous copy, which
+    # overlaps the previous site's kernels
+    up = runtime.upload_many(
+        [np.ascontiguousarray(Sigma, dtype=np.complex128), t.sqrt_fact, _i32(left.targets), _i32(left.perm),
+         _i32(right.targets), _i32(right.perm), left.den, right.den, t.lam])
+    (sig_d, sf_d, lt_d, lp_d, rt_d, rp_d, lden_d, rden_d, lam_d) = up
+    c = SiteCall()
+    c.k, c.tables, c.uploads, c.Lambda = k, t, up, t.Lambda
+    c.lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
+    c.rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
+    c.args = _lib.GammaSiteArgs(
+        kind=t.kind, d=d, chi=t.n_a, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
+        sqrt_fact=sf_d.data_ptr(),
+        n_left=len(left.count), w_left=left.targets.shape[1], left_targets=lt_d.data_ptr(),
+        left_den=lden_d.data_ptr(), lambda_=lam_d.data_ptr(), left_perm=lp_d.data_ptr(),
+        left_class_count_host=c.lcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_left_class=len(c.lcc),
+        n_right=len(right.count), w_right=right.targets.shape[1], right_targets=rt_d.data_ptr(),
+        right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
+        right_class_count_host=c.rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(c.rcc))
+    c.gamma = torch.zeros((t.n_b, t.n_a, d), dtype=torch.complex128, device=dev)
+    return c
+
+
+def gamma_sites(calls, timer=None):
+    """Enqueue the site tensors of ``calls`` (SiteCalls of one instance or of several) in one
+    ``bsmps_gamma_sites`` call: one hafnian launch per photon total covers every listed site."""
+    calls = list(calls)
+    if not calls:
+        return
+    n = len(calls)
+    arr = (_lib.GammaSiteArgs * n)(*[c.args for c in calls])
+    outs = (ctypes.c_void_p * n)(*[runtime.ptr(c.gamma) for c in calls])
+    with (timer("gamma_sites") if timer else contextlib.nullcontext()):
+        runtime.context().call("bsmps_gamma_sites", n, arr, outs)
+
+
 def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, err_tol=ERR_TOL, timer=None,
                 sigma_z=None, ranks=None, tables=None, right_runs=1, on_run=None):
     """Gamma_k as a device tensor [chi, chi, d] complex128 and Lambda_k (float64 [chi]).
@@ -253,42 +310,22 @@ def site_tensor(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full=None, e
     [rank_{k-1}][rank_k][d] and Lambda_k has rank_k entries (one row / one column at the
     chain ends)."""
     ctx = runtime.context()
-    dev = ctx.device
-    Sigma, Z = sigma_z if sigma_z is not None else gaussian.site_kernel(k, M, sq_cov, kron_S, S_full)
-    t = tables if tables is not None else site_tables(k, M, d, chi, kron_res, kron_num, err_tol, ranks)
-    Mp = Sigma.shape[0] // 2
-    if k > 0 and Mp != M - k:
-        raise ValueError(f"site {k}: Sigma has {Mp} modes, expected {M - k}")
-    left, right = t.left, t.right
-    # every table of the site in one pinned staging buffer and one asynchronous copy, which
-    # overlaps the previous site's kernels
-    (sig_d, sf_d, lt_d, lp_d, rt_d, rp_d, lden_d, rden_d, lam_d) = runtime.upload_many(
-        [np.ascontiguousarray(Sigma, dtype=np.complex128), t.sqrt_fact, _i32(left.targets), _i32(left.perm),
-         _i32(right.targets), _i32(right.perm), left.den, right.den, t.lam])
-    lcc = np.ascontiguousarray(left.class_count, dtype=np.int32)
-    rcc = np.ascontiguousarray(right.class_count, dtype=np.int32)
-    args = _lib.GammaSiteArgs(
-        kind=t.kind, d=d, chi=t.n_a, ld=2 * Mp, sigma=sig_d.data_ptr(), z=Z, inv_z=1.0 / Z,
-        sqrt_fact=sf_d.data_ptr(),
-        n_left=len(left.count), w_left=left.targets.shape[1], left_targets=lt_d.data_ptr(),
-        left_den=lden_d.data_ptr(), lambda_=lam_d.data_ptr(), left_perm=lp_d.data_ptr(),
-        left_class_count_host=lcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_left_class=len(lcc),
-        n_right=len(right.count), w_right=right.targets.shape[1], right_targets=rt_d.data_ptr(),
-        right_den=rden_d.data_ptr(), right_perm=rp_d.data_ptr(),
-        right_class_count_host=rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n_right_class=len(rcc))
-    gamma = torch.zeros((t.n_b, t.n_a, d), dtype=torch.complex128, device=dev)
-    runs = right_label_runs(t, d, right_runs)
+    c = site_call(k, M, sq_cov, kron_res, kron_num, kron_S, d, chi, S_full, err_tol, sigma_z, ranks, tables)
+    right = c.tables.right
+    args = c.args
+    rp_ptr = args.right_perm
+    runs = right_label_runs(c.tables, d, right_runs)
     for i, (r0, r1) in enumerate(runs):
         if len(runs) > 1:  # this run's labels: a slice of the class-ordered permutation
             rcc = np.bincount(right.count[right.perm[r0:r1]], minlength=len(right.class_count)).astype(np.int32)
             args.n_right = r1 - r0
-            args.right_perm = rp_d.data_ptr() + 4 * r0
+            args.right_perm = rp_ptr + 4 * r0
             args.right_class_count_host = rcc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         with (timer("gamma_site") if timer else contextlib.nullcontext()):
-            ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(gamma))
+            ctx.call("bsmps_gamma_site", ctypes.byref(args), runtime.ptr(c.gamma))
         if on_run is not None:
             on_run(i, len(runs))
-    return gamma, t.Lambda
+    return c.gamma, c.Lambda
 
 
 class HostStream:
@@ -401,6 +438,39 @@ def build_mps(sq_cov, d, chi, kron=None, max_dim=10 ** 5, timer=None, site_d=Non
     if streamer is not None:
         streamer.drain()
     return Gammas, Lambdas, kron
+
+
+def site_calls(sq_cov, d, chi, kron=None, max_dim=10 ** 5, pruned=False, site_d=None, site_chi=None):
+    """The SiteCalls of every site of one instance, ready for ``gamma_sites`` (alone or merged
+    with other instances'): the Schmidt spectra run on the GPU (unless ``kron`` is given), the
+    host FP64 work and the label tables in the worker pool, and each site's tables are uploaded;
+    no hafnian runs yet.  ``pruned`` / ``site_d`` / ``site_chi`` as in build_mps.  Returns
+    (calls, kron)."""
+    from . import kron as kron_mod
+    M = len(sq_cov) // 2
+    S_full_f = hostpool.williamson_S(sq_cov)
+    if kron is None:
+        cuts = hostpool.cut_inputs_all(sq_cov, d)
+        kS = [S for _, S in cuts]
+        site_f = hostpool.submit_sites(sq_cov, kS, S_full_f.result())
+        spec = kron_mod.kron_cuts([tp for tp, _ in cuts], chi, max_dim)
+        kron = ([r for r, _ in spec], [n for _, n in spec], kS)
+    else:
+        site_f = hostpool.submit_sites(sq_cov, kron[2], S_full_f.result())
+    kres, knum, kS = kron
+    shaped = site_d is not None or site_chi is not None or pruned
+    if shaped:
+        site_d = list(site_d) if site_d is not None else [d] * M
+        if site_chi is None:
+            site_chi = [int((np.asarray(r) > ERR_TOL).sum()) for r in kres]
+    tab_f = hostpool.submit_tables(M, d, chi, kres, knum, ERR_TOL, site_d if shaped else None,
+                                   site_chi if shaped else None)
+    calls = []
+    for k in range(M):
+        ranks = (site_chi[k - 1] if k > 0 else None, site_chi[k] if k < M - 1 else None) if shaped else None
+        calls.append(site_call(k, M, sq_cov, kres, knum, kS, site_d[k] if shaped else d, chi,
+                               sigma_z=site_f[k].result(), ranks=ranks, tables=tab_f[k].result_or_run()))
+    return calls, kron
 
 
 def _save_atomic(path, arr):

@@ -145,6 +145,14 @@ typedef struct {
 
 int bsmps_gamma_site(bsmps_ctx *ctx, const bsmps_gamma_site_args *args, double *gamma);
 
+/* Many site tensors at once, of one instance or of several: args[i] describes site i and
+ * gammas[i] is its output (zeroed by the caller), as for bsmps_gamma_site.  Per even photon total
+ * one hafnian launch covers the (j, a, b) items of every listed site, so sites and instances with
+ * uneven cutoffs, ranks and hafnian sizes share launches.  The values are those of
+ * bsmps_gamma_site, bit for bit.  Replaces the per-site loop of MPS_cpu.py:384-525 and the
+ * site-per-rank offload of distributed_MPS.py:188-246 for batches of independent instances. */
+int bsmps_gamma_sites(bsmps_ctx *ctx, int n_sites, const bsmps_gamma_site_args *args, double *const *gammas);
+
 /* ---- Schmidt spectra (L2) -------------------------------------------------------------- */
 
 /* The pruned Kronecker loop of get_cumsum_kron, kron_cpu.py:106-125 (GPU path
