@@ -594,6 +594,8 @@ def main():
                     help="with --partial-sites: build the sites in the learned-cutoff form")
     ap.add_argument("--partial-sample", type=int, default=0,
                     help="with --partial-sites including M//2: time that site's sampling step for this many samples")
+    ap.add_argument("--batched-eager", action="store_true",
+                    help="with --batched: run each sampling pass launch by launch instead of as a HIP graph")
     ap.add_argument("--batched", type=int, default=0,
                     help="run the batched form over this many random instances (graph-captured sampling)")
     ap.add_argument("--hafnian-size", type=int, default=0,
@@ -643,7 +645,8 @@ def main():
         if not args.no_learned:
             from bosonsupremacy_amd import learned
             model = learned.load_or_train()
-        out = batched.run_batch_merged(insts, n_samples=args.samples or 10 ** 4, verbose=True, learned_model=model)
+        out = batched.run_batch_merged(insts, n_samples=args.samples or 10 ** 4, verbose=True, learned_model=model,
+                                       graphs=not args.batched_eager)
         out.update({"metric": "instances/s (merged construction + one graph-replayed sampling pass per form)",
                     "workload": f"{args.batched} random instances, M~U{{16..96}}, d~U{{4..8}}, chi~U{{128..1024}}",
                     "tuning": {str(k): v for k, v in tuning.items()}})

@@ -212,7 +212,25 @@ def run_batch(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), 
     return out
 
 
-def _pass_and_errors(inst, G, L, sqrtW, names, n_samples, now, plan=None, learned_precision="fp16x3"):
+class EagerPass:
+    """A Sampler run launch by launch on the process context, with GraphSampler's interface:
+    for a pass that is run once, capturing it costs more than its launches (the warm-up pass and
+    the capture of an FP64 pass took ~100 ms per instance of the batched workload, its launches a
+    few)."""
+
+    def __init__(self, gammas, lambdas, sqrtW, dd, n, precision="fp64", return_probs=False):
+        t0 = time.perf_counter()
+        self.sampler = sampling.Sampler(gammas, lambdas, sqrtW, dd, precision=precision)
+        torch.cuda.synchronize()
+        self.return_probs = return_probs
+        self.prepare_s, self.capture_s = time.perf_counter() - t0, 0.0
+
+    def run(self, normals, uniforms):
+        return self.sampler.run(normals, uniforms, return_probs=self.return_probs)
+
+
+def _pass_and_errors(inst, G, L, sqrtW, names, n_samples, now, plan=None, learned_precision="fp16x3",
+                     graphs=True):
     """One instance's sampling in every form of ``names`` (the first is FP64): a graph per form,
     one replay with the Philox uniforms (timed), and each reduced form replayed along the FP64
     sample paths (mid-interval uniforms).  ``plan`` = (site_d, site_chi) adds the learned-cutoff
@@ -233,13 +251,16 @@ def _pass_and_errors(inst, G, L, sqrtW, names, n_samples, now, plan=None, learne
             Ll.append(None if L[k] is None else np.asarray(L[k])[:cols])
         forms.append("learned")
         shapes["learned"] = (Gl, Ll)
+    runner = GraphSampler if graphs else EagerPass
     graphs = {}
     for f in forms:
         ta = now()
         g, lam = shapes[f]
-        graphs[f] = GraphSampler(g, lam, sqrtW, inst.dd, n_samples, learned_precision if f == "learned" else f,
-                                 return_probs=True)
+        graphs[f] = runner(g, lam, sqrtW, inst.dd, n_samples, learned_precision if f == "learned" else f,
+                           return_probs=True)
         times["setup_" + f] = now() - ta
+        times["prepare_" + f] = graphs[f].prepare_s
+        times["capture_" + f] = graphs[f].capture_s
     normals, uniforms = sampling.random_inputs(inst.M, n_samples, seed=inst.seed)
     ta = now()
     s64, p64 = graphs[forms[0]].run(normals, uniforms)
@@ -266,7 +287,7 @@ def _pass_and_errors(inst, G, L, sqrtW, names, n_samples, now, plan=None, learne
 
 
 def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp16x3"), group_bytes=32 << 30,
-                     verbose=False, learned_model=None, learned_precision="fp16x3"):
+                     verbose=False, learned_model=None, learned_precision="fp16x3", graphs=True):
     """Batched form with cross-instance construction.  Instances are taken in groups whose site
     tensors together fit ``group_bytes``; the site tensors of a whole group -- every site of
     every instance in it -- are built by one ``bsmps_gamma_sites`` call, i.e. one hafnian launch
@@ -279,7 +300,8 @@ def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp1
     with relative errors of the Rao-Blackwellised marginals.  With ``learned_model`` each
     instance's exactly judged plan is walked as one more form on the leading blocks of its
     pruned tensors (the tensors that plan would build), so its construction is the pruned one.
-    Instances whose hafnians exceed the library's sizes are skipped and listed.
+    Instances whose hafnians exceed the library's sizes are skipped and listed.  ``graphs=False``
+    runs each pass launch by launch (EagerPass) instead of capturing it.
 
     Timing: the sampling of each instance (graphs, passes, error replays) is timed as a block;
     construction is the rest of the wall time.  instances/s of a form = instances /
@@ -294,7 +316,7 @@ def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp1
     all_forms = names + (("learned",) if learned_model is not None else ())
     tot = {f: 0.0 for f in ("check", "prepare", "merged_launch")}
     for f in all_forms:
-        tot["setup_" + f] = tot["sampling_" + f] = 0.0
+        tot["setup_" + f] = tot["sampling_" + f] = tot["prepare_" + f] = tot["capture_" + f] = 0.0
     errs = {f: [] for f in all_forms[1:]}
     rows, skipped, group_sizes = [], [], []
     side = torch.cuda.Stream()
@@ -362,7 +384,7 @@ def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp1
                                                              inst.d, inst.chi)
                 plan = (site_d, site_chi)
             times, row, e = _pass_and_errors(inst, G, L, sampling.thermal_sqrtW(inst.cov, inst.sq_cov), names,
-                                             n_samples, now, plan, learned_precision)
+                                             n_samples, now, plan, learned_precision, graphs)
             for k, v in times.items():
                 tot[k] += v
             for f, v in e.items():
@@ -387,6 +409,7 @@ def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp1
     out = {"instances": done, "skipped": skipped, "samples_per_instance": n_samples, "wall_s": wall,
            "construction": "merged: one bsmps_gamma_sites call per group of instances, the next group "
                            "prepared on the host meanwhile",
+           "passes": "one HIP graph per instance and form" if graphs else "launch by launch",
            "groups": len(group_sizes), "instances_per_group": group_sizes, "construction_s": construction,
            "host_prepare_s": tot["prepare"], "sampling_and_checks_s": tot["check"]}
     for f in all_forms:
@@ -399,6 +422,8 @@ def run_batch_merged(instances, n_samples=10 ** 4, forms=("fp32", "bf16x6", "fp1
         out[f"{f}_second_order_rel_err_max"] = float(np.max([x[2] for x in e])) if e else 0.0
         out[f"{f}_off_fp64_path_total"] = int(sum(r.get(f"{f}_off_fp64_path", 0) for r in rows))
     out["graph_setup_s"] = {f: tot["setup_" + f] for f in all_forms}
+    out["graph_setup_split_s"] = {f: {"form_preparation": tot["prepare_" + f], "warm_up_and_capture": tot["capture_" + f]}
+                                  for f in all_forms}
     out["per_instance"] = rows
     return out
 

@@ -76,3 +76,16 @@ def test_run_batch_merged_learned_form():
     assert out["instances_per_s_learned"] > 0
     assert out["learned_marginal_abs_err_max"] < 1e-6
     assert out["learned_off_fp64_path_total"] <= out["instances"] * 500 // 100
+
+
+def test_run_batch_merged_eager_passes_match_graphs():
+    """Passes run launch by launch give the same draws and errors as the graph-replayed ones."""
+    from bosonsupremacy_amd import batched
+    insts = small_batch(2, seed=5)
+    a = batched.run_batch_merged(insts, n_samples=400, forms=("fp16x3",))
+    b = batched.run_batch_merged(insts, n_samples=400, forms=("fp16x3",), graphs=False)
+    assert a["instances"] == b["instances"] >= 1
+    for ra, rb in zip(a["per_instance"], b["per_instance"]):
+        for key in ("fp16x3_marginal_abs_err", "fp16x3_first_order_rel_err", "fp16x3_second_order_rel_err",
+                    "fp16x3_off_fp64_path"):
+            assert ra[key] == rb[key], key
