@@ -951,7 +951,7 @@ __global__ __launch_bounds__(256) void site_displace_kernel(int64_t n, int d, in
 // chain cannot underflow -- by 1/sqrt(p_pick), which makes its lam-weighted norm 1; the
 // reference's GPU path rescales by the largest component, distributed_sampling.py:147, and the
 // draws only see ratios of probabilities either way).  The probabilities and the inverse CDF are
-// evaluated in FP64 in every form.  Used for d > 8 or dd > 16 (sample_site_gram_kernel below
+// evaluated in FP64 in every form.  Used for d > 10 or dd > 20 (sample_site_gram_kernel below
 // covers the smaller cutoffs).
 template <int DMAX, int DDMAX, typename TT, bool RENORM>
 __global__ __launch_bounds__(256) void sample_site_kernel(int64_t n, int chi, int d, int dd,
@@ -1078,8 +1078,8 @@ __device__ __forceinline__ double row16_sum(double v) {
     return v;
 }
 
-// Probabilities + draw + carried vector through the per-sample Gram matrix, for d <= 8 and
-// dd <= 16: one workgroup of 256 threads per sample.  The sample's rows of T are staged through
+// Probabilities + draw + carried vector through the per-sample Gram matrix, for d <= 10 and
+// dd <= 20: one workgroup of 256 threads per sample.  The sample's rows of T are staged through
 // LDS in chunks of R rows with coalesced loads (consecutive threads read consecutive elements;
 // row a's element j lands at a * RS + j, RS = d rounded up to odd against bank conflicts), and
 // when all chi rows fit (R >= chi) the carried-vector pass reads them from LDS again instead of
@@ -1104,7 +1104,11 @@ struct GramStage<float> {
     using V = float2;
 };
 
-template <int DMAX, int DDMAX, typename TT, bool RENORM>
+// SPLIT = 2 (d <= 10): the workgroup's two halves -- waves 0-1 and 2-3 -- accumulate the first
+// and the second half of the G entries over the same rows, so each thread holds half of them
+// (55 complex entries at d = 10 would not fit one thread's registers); each half's 8 row
+// partials of an entry are then added in a fixed order.
+template <int DMAX, int DDMAX, typename TT, bool RENORM, int SPLIT = 1>
 __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int chi, int d, int dd,
                                                                const TT* __restrict__ T, int64_t tstride,
                                                                const double* __restrict__ Dg,
@@ -1114,8 +1118,12 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
                                                                TT* __restrict__ pre, double* __restrict__ probs,
                                                                int R, int RS, unsigned dmagic) {
     static_assert(DDMAX <= 64 && DMAX <= DDMAX, "one candidate photon number per thread");
+    static_assert(SPLIT == 1 || SPLIT == 2, "the G entries are held by one or two halves of the workgroup");
     using V = typename GramStage<TT>::V;
     constexpr int NP = DMAX * (DMAX + 1) / 2;
+    constexpr int NPH = (NP + SPLIT - 1) / SPLIT;  // G entries per thread
+    constexpr int RSTEP = 256 / SPLIT;             // rows walked in parallel by one half
+    constexpr int WPH = 16 / SPLIT;                // 16-lane rows per half
     __shared__ V s_T[GramStage<TT>::kBytes / sizeof(V)];
     __shared__ double s_part[16][2 * NP];  // per 16-lane row: G partial (re, im per entry)
     __shared__ double s_G[2 * NP];
@@ -1123,6 +1131,7 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
     __shared__ double s_scale;
     __shared__ int s_pick;
     const int tid = threadIdx.x, r16 = tid % 16, row = tid / 16;
+    const int half = tid / RSTEP, rtid = tid % RSTEP;  // wave-uniform half
     // rows [r0, r0 + nr) of the sample into s_T, four loads in flight per thread (more loads
     // per batch were measured slower); the caller synchronises around it
     auto stage = [&](const V* Tv, int r0, int nr) {
@@ -1158,16 +1167,16 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
     };
     for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
         const V* Tv = reinterpret_cast<const V*>(T) + s * tstride;
-        double gr[NP], gi[NP];
+        double gr[NPH], gi[NPH];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) gr[p] = gi[p] = 0.0;
+        for (int p = 0; p < NPH; ++p) gr[p] = gi[p] = 0.0;
         const int a_end = lam ? chi : 1;
         for (int r0 = 0; r0 < a_end; r0 += R) {
             const int nr = a_end - r0 < R ? a_end - r0 : R;
             __syncthreads();  // the previous chunk / sample is no longer read
             stage(Tv, r0, nr);
             __syncthreads();
-            for (int aa = tid; aa < nr; aa += 256) {
+            for (int aa = rtid; aa < nr; aa += RSTEP) {
                 const double l = lam ? lam[r0 + aa] : 1.0;
                 cd t[DMAX];
                 row_of(aa, t);
@@ -1176,13 +1185,14 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
                 for (int j = 0; j < DMAX; ++j) {
 #pragma unroll
                     for (int q = j; q < DMAX; ++q, ++p) {
-                        if (q < d) {
+                        if (q < d && p / NPH == half) {
                             const cd x = cscale(t[j], l), y = cscale(t[q], l);
+                            const int i = p % NPH;
                             if (q == j) {
-                                gr[p] = fma(x.re, x.re, fma(x.im, x.im, gr[p]));
+                                gr[i] = fma(x.re, x.re, fma(x.im, x.im, gr[i]));
                             } else {  // t_j conj(t_q)
-                                gr[p] = fma(x.re, y.re, fma(x.im, y.im, gr[p]));
-                                gi[p] = fma(x.im, y.re, fma(-x.re, y.im, gi[p]));
+                                gr[i] = fma(x.re, y.re, fma(x.im, y.im, gr[i]));
+                                gi[i] = fma(x.im, y.re, fma(-x.re, y.im, gi[i]));
                             }
                         }
                     }
@@ -1195,9 +1205,9 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
             for (int j = 0; j < DMAX; ++j) {
 #pragma unroll
                 for (int q = j; q < DMAX; ++q, ++p) {
-                    if (q < d) {
-                        const double vr = row16_sum(gr[p]);
-                        const double vi = q != j ? row16_sum(gi[p]) : 0.0;
+                    if (q < d && p / NPH == half) {
+                        const double vr = row16_sum(gr[p % NPH]);
+                        const double vi = q != j ? row16_sum(gi[p % NPH]) : 0.0;
                         if (r16 == 0) {
                             s_part[row][2 * p] = vr;
                             s_part[row][2 * p + 1] = vi;
@@ -1207,10 +1217,11 @@ __global__ __launch_bounds__(256) void sample_site_gram_kernel(int64_t n, int ch
             }
         }
         __syncthreads();
-        if (tid < 2 * NP) {
+        if (tid < 2 * NP) {  // entry tid / 2 was accumulated by the 16-lane rows of its half
+            const int w0 = (tid / 2) / NPH * WPH;
             double acc = 0.0;
 #pragma unroll
-            for (int w = 0; w < 16; ++w) acc += s_part[w][tid];
+            for (int w = 0; w < WPH; ++w) acc += s_part[w0 + w][tid];
             s_G[tid] = acc;
         }
         __syncthreads();
@@ -1441,16 +1452,18 @@ extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int 
 #define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
     hipLaunchKernelGGL((sample_site_kernel<DM, DDM, double, false>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
                        T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
-#define BSMPS_GRAM_LAUNCH(DM, DDM)                                                                              \
-    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, double, false>), grid16, block, 0, ctx->stream, n, chi, d, dd, \
-                       T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS, dmagic)
+#define BSMPS_GRAM_LAUNCH(DM, DDM, SPL)                                                                         \
+    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, double, false, SPL>), grid16, block, 0, ctx->stream, n, chi, \
+                       d, dd, T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS,  \
+                       dmagic)
     const dim3 grid16(grid_for(n, 1));
     int R = 0, RS = 0;
     unsigned dmagic = 0;
     gram_stage_shape<double>(chi, d, R, RS, dmagic);
-    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8);
-    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12);
-    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16);
+    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8, 1);
+    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12, 1);
+    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16, 1);
+    else if (d <= 10 && dd <= 20) BSMPS_GRAM_LAUNCH(10, 20, 2);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
 #undef BSMPS_GRAM_LAUNCH
@@ -1529,16 +1542,18 @@ extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, 
 #define BSMPS_SAMPLE_LAUNCH(DM, DDM)                                                                           \
     hipLaunchKernelGGL((sample_site_kernel<DM, DDM, float, true>), grid, block, 0, ctx->stream, n, chi, d, dd, T, \
                        T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs)
-#define BSMPS_GRAM_LAUNCH(DM, DDM)                                                                              \
-    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, float, true>), grid16, block, 0, ctx->stream, n, chi, d, dd, \
-                       T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS, dmagic)
+#define BSMPS_GRAM_LAUNCH(DM, DDM, SPL)                                                                         \
+    hipLaunchKernelGGL((sample_site_gram_kernel<DM, DDM, float, true, SPL>), grid16, block, 0, ctx->stream, n, chi, \
+                       d, dd, T, T_sample_stride, dwork, lambda, uniforms, samples, samples_stride, pre, probs, R, RS, \
+                       dmagic)
     const dim3 grid16(grid_for(n, 1));
     int R = 0, RS = 0;
     unsigned dmagic = 0;
     gram_stage_shape<float>(chi, d, R, RS, dmagic);
-    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8);
-    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12);
-    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16);
+    if (d <= 4 && dd <= 8) BSMPS_GRAM_LAUNCH(4, 8, 1);
+    else if (d <= 6 && dd <= 12) BSMPS_GRAM_LAUNCH(6, 12, 1);
+    else if (d <= 8 && dd <= 16) BSMPS_GRAM_LAUNCH(8, 16, 1);
+    else if (d <= 10 && dd <= 20) BSMPS_GRAM_LAUNCH(10, 20, 2);
     else if (d <= 16 && dd <= 32) BSMPS_SAMPLE_LAUNCH(16, 32);
     else BSMPS_SAMPLE_LAUNCH(32, 64);
 #undef BSMPS_GRAM_LAUNCH

@@ -37,6 +37,7 @@ def _reference(T, alpha, lam, u, dd):
 CASES = [  # (n, chi, d, dd, last)
     (96, 300, 10, 20, False), (64, 64, 16, 40, False), (40, 1200, 8, 16, False), (48, 600, 6, 12, False),
     (80, 100, 4, 8, False), (50, 1, 10, 20, True), (50, 1, 6, 12, True),
+    (32, 2000, 10, 20, False), (40, 500, 9, 18, False), (30, 700, 10, 16, False),
 ]
 
 
