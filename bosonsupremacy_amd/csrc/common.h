@@ -41,6 +41,16 @@ __device__ __forceinline__ cd ld_cd(const double* p, long i) {
 __device__ __forceinline__ void st_cd(double* p, long i, cd v) {
     reinterpret_cast<double2*>(p)[i] = make_double2(v.re, v.im);
 }
+// Streaming forms (non-temporal cache policy) for data a kernel reads or writes once, so that
+// it does not displace the data other work items of the same launch read again from the L2.
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cd ld_cd_nt(const double* p, long i) {
+    const dbl2_t v = __builtin_nontemporal_load(reinterpret_cast<const dbl2_t*>(p) + i);
+    return cd{v.x, v.y};
+}
+__device__ __forceinline__ void st_cd_nt(double* p, long i, cd v) {
+    __builtin_nontemporal_store(dbl2_t{v.re, v.im}, reinterpret_cast<dbl2_t*>(p) + i);
+}
 
 // Host-side error plumbing shared by every translation unit.
 void bsmps_set_error(const char* fmt, ...);

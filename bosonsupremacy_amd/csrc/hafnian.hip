@@ -732,10 +732,15 @@ __device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int
         }
         // The entry's own coefficients are all loaded before the first store: loads and stores
         // share one buffer, so a load placed after a store cannot be moved above it, and each
-        // coefficient would then wait a full memory latency of its own.
+        // coefficient would then wait a full memory latency of its own.  They are read by this
+        // item only, and the child coefficients below are written once, so both stream past the
+        // L2 (non-temporal), which keeps the pivot polynomials every item of the node re-reads.
         cd acc[D2 + 1];
 #pragma unroll
-        for (int t = 1; t <= D2; ++t) acc[t] = (t <= tmax && t <= D) ? co(J, K, t) : czero();
+        for (int t = 1; t <= D2; ++t)
+            acc[t] = (t <= tmax && t <= D)
+                         ? ld_cd_nt(bd.data, (int64_t)(op.po + tri(J - ob, K - ob) * op.pcap + (t - 1)) * bd.stride + h)
+                         : czero();
         const int64_t dst = (int64_t)(op.pdst + e * op.dcap) * bd.stride + h;
 #pragma unroll
         for (int t = 1; t <= D2; ++t) {
@@ -747,7 +752,7 @@ __device__ __forceinline__ void bfs_item(const BfsData& bd, const BfsOp& op, int
                     const int v = t - 1 - u;
                     if (v >= 0 && v <= D && u <= D) acc[t] = cadd(acc[t], cadd(cmul(X[u], Y[v]), cmul(Z[u], W[v])));
                 }
-                st_cd(bd.data, dst + (int64_t)(t - 1) * bd.stride, acc[t]);
+                st_cd_nt(bd.data, dst + (int64_t)(t - 1) * bd.stride, acc[t]);
             }
         }
     } else {
