@@ -100,6 +100,26 @@ def site_hafnian_work_by_site(kres, knum, d, err_tol=1e-10, site_d=None, site_ch
     return out
 
 
+def device_copy_gbs(nbytes=4 << 30, reps=5):
+    """GB/s (bytes read + written) of torch's device-to-device copy of nbytes: what a plain
+    streaming kernel with a half-read, half-write mix reaches on this GPU."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (a.elapsed_time(b) / 1e3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def probe_sampling(inst, ko, probe, n, tuning):
     """Time one site's sampling step at full size: the contraction of n carried vectors with
     a real Gamma_k (FP64, FP32, BF16x6 and FP16x3 forms) and the displacement + draw, then
@@ -830,6 +850,12 @@ def standard_run(args):
     roof["traffic"] = tr["bytes_per_call"] if tr else None
     if tr:
         roof["traffic_source"] = f"{tr['source']} (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE per call, not this run)"
+        rate = tr["bytes_per_call"] / (kernels[dom]["mean_ms"] / 1e3) / 1e9
+        copy_gbs = device_copy_gbs()
+        roof["hbm"] = {"rate_GBps": rate, "peak_GBps": PEAK_HBM_GBS, "frac_of_peak": rate / PEAK_HBM_GBS,
+                       "device_copy_GBps": copy_gbs, "frac_of_device_copy": rate / copy_gbs,
+                       "note": "profiled HBM bytes per call / this run's mean call time; device_copy = a 4 GiB "
+                               "device-to-device copy (half read, half write) timed in this run on the same GPU"}
     log(f"timed steps done at {time.perf_counter() - t_begin:.0f} s")
 
     # ---- precision forms along the FP64 sample paths -----------------------------------------
