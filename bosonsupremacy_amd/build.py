@@ -22,6 +22,10 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
+# The split-form contraction keeps its accumulators across a two-stage k loop; left to its
+# heuristics the register allocator puts them in VGPRs for one stage and AGPRs for the other
+# and copies all 64 across twice per iteration.  The VGPR form of the MFMA operands avoids it.
+SOURCE_FLAGS = {"sampling.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form"]}
 
 
 def _hipcc():
@@ -36,7 +40,7 @@ def build(verbose=False, extra_flags=()):
     procs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [_hipcc(), *HIPCC_FLAGS, *extra_flags, "-I", INCLUDE, "-c",
+        cmd = [_hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), *extra_flags, "-I", INCLUDE, "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

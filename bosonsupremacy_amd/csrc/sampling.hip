@@ -569,8 +569,7 @@ struct Bf16x6Fmt {  // three bf16 parts (exact), products of order >= 2^-16: six
     typedef bf16x8 V;
     static constexpr int NP = 3;
     static constexpr bool SCALED = false;
-    static constexpr int MIN_WAVES = 2;  // waves per SIMD the registers are budgeted for
-    static constexpr bool PREFETCH_B = false;  // B fetched one step ahead spills at 6 parts
+    static constexpr bool DMA = true;  // staged by LDS DMA (site_gemm_split_dma_kernel)
     __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
     __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
         acc = mfma(a[2], b[0], acc);
@@ -587,8 +586,7 @@ struct Fp16x3Fmt {  // two fp16 parts (22 bits) of values scaled to max 1: three
     typedef half8 V;
     static constexpr int NP = 2;
     static constexpr bool SCALED = true;
-    static constexpr int MIN_WAVES = 2;  // at 3 the kernel spills
-    static constexpr bool PREFETCH_B = true;  // B fetched one step ahead, like A
+    static constexpr bool DMA = false;  // staged through registers (site_gemm_split_kernel)
     __device__ static float4_t mfma(V a, V b, float4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
     __device__ static float4_t prod(const V (&a)[NP], const V* b, float4_t acc) {
         acc = mfma(a[1], b[0], acc);
@@ -676,7 +674,7 @@ __global__ __launch_bounds__(256) void split_k_combine_kernel(const float* __res
     }
 }
 
-// One kernel for both split formats: F::NP parts per value, 2 F::NP planes per operand (parts of
+// Split-form contraction kernels: F::NP parts per value, 2 F::NP planes per operand (parts of
 // the real part, then of the imaginary part).  With F::SCALED the planes hold values divided by a
 // per-row (A) and per-column (B) scale, which the epilogue multiplies back.
 //
@@ -685,16 +683,96 @@ __global__ __launch_bounds__(256) void split_k_combine_kernel(const float* __res
 // T + s * (n * ncols * 2) floats; split_k_combine_kernel then adds the partials in FP64.  A shorter
 // FP32 accumulation chain per output is the point (at k = 10^4 the ~2000 FP32 roundings of one
 // chain, not the products, set the error).  kseg = kpad is the unsegmented product.
+//
+// Two staging forms share the tile (128 rows x 64 complex columns x 32 complex k, four waves of
+// 64 x 32), the LDS layout, the MFMA sequence of every accumulator (so both give the same bits)
+// and the epilogue:
+//   * registers (site_gemm_split_kernel, FP16x3): one LDS stage, the next k step fetched into
+//     registers while this one is multiplied, two barriers per step, two workgroups per CU;
+//   * LDS DMA (site_gemm_split_dma_kernel, BF16x6): two LDS stages filled straight from global
+//     memory (global_load_lds_dwordx4), one barrier per step, one workgroup per CU.
+// Each step has a fixed cost of about 800 cycles (barrier, the wait for the stage, the first
+// fragment reads) that the second workgroup of the register form hides and the DMA form does
+// not: the DMA form is 1.3x faster for BF16x6 (192 MFMAs per wave and step, and no spills in
+// its register budget) and 10% slower for FP16x3 (96 MFMAs per wave and step; also with three
+// stages, two steps in flight).
+
+// Fragments of one k step from a stage in LDS ([plane][row][slot] of 16 bytes, slot x6_slot) into
+// the wave's accumulators: lane holds k 8 (lane / 16) .. + 8 of row / column lane % 16.
 template <class F>
-__global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+__device__ __forceinline__ void split_multiply(const typename F::E* As, const typename F::E* Bs, int wm, int wn,
+                                               int lane, float4_t (&acc_re)[4][2], float4_t (&acc_im)[4][2]) {
+    typedef typename F::V V;
+    constexpr int NP = F::NP, NPL = 2 * F::NP;
+    const int g = lane / 16;
+    V bfr[2][NPL];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int c = wn + 16 * j + lane % 16;
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl)
+            bfr[j][pl] = *reinterpret_cast<const V*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = wm + 16 * i + lane % 16;
+        V ar[NP], ai[NP], nai[NP];
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            ar[t] = *reinterpret_cast<const V*>(As + ((t * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+            ai[t] = *reinterpret_cast<const V*>(As + (((t + NP) * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
+            nai[t] = neg16x8(ai[t]);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = F::prod(ar, &bfr[j][0], acc_re[i][j]);
+            acc_re[i][j] = F::prod(nai, &bfr[j][NP], acc_re[i][j]);
+            acc_im[i][j] = F::prod(ar, &bfr[j][NP], acc_im[i][j]);
+            acc_im[i][j] = F::prod(ai, &bfr[j][0], acc_im[i][j]);
+        }
+    }
+}
+
+// Epilogue: output register r of lane l holds row 4 (l / 16) + r, column l % 16 of each block.
+template <class F>
+__device__ __forceinline__ void split_store(const float4_t (&acc_re)[4][2], const float4_t (&acc_im)[4][2],
+                                            int64_t n, int ncols, int64_t row0, int col0, int wm, int wn, int lane,
+                                            const float* __restrict__ rowscale, const float* __restrict__ colscale,
+                                            float* __restrict__ T) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t srow = row0 + wm + 16 * i + 4 * (lane / 16) + r;
+                const int c = col0 + wn + 16 * j + lane % 16;
+                if (srow < n && c < ncols) {
+                    float re = acc_re[i][j][r], im = acc_im[i][j][r];
+                    if constexpr (F::SCALED) {
+                        const float sc = rowscale[srow] * colscale[c];
+                        re *= sc;
+                        im *= sc;
+                    }
+                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(re, im);
+                }
+            }
+}
+
+// Register staging.  Pieces of 16 bytes: piece w = 4 pl + g of row r is the NPL * 64-byte block
+// of the step (x6_at).  A: 128 rows x 4 NPL pieces (2 NPL per thread, fetched into registers one
+// step ahead), B: 64 columns x 4 NPL pieces (NPL per thread, fetched one step ahead as well);
+// consecutive threads read consecutive pieces.  Rows past n read row n - 1 (their results are
+// not stored).
+template <class F>
+__global__ __launch_bounds__(256, 2) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
                                                                  const typename F::E* __restrict__ apl,
                                                                  const float* __restrict__ rowscale,
                                                                  const typename F::E* __restrict__ bpl,
                                                                  const float* __restrict__ colscale,
                                                                  float* __restrict__ T, int kseg) {
     typedef typename F::E E;
-    typedef typename F::V V;
-    constexpr int NP = F::NP, NPL = 2 * F::NP;
+    constexpr int NPL = 2 * F::NP;
     __shared__ __attribute__((aligned(16))) E As[NPL * X6_TM * X6_KC];
     __shared__ __attribute__((aligned(16))) E Bs[NPL * X6_TN * X6_KC];
     int rb, cb;
@@ -715,22 +793,8 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-    // staging pieces of 16 bytes: piece w = 4 pl + g of row r is the NPL * 64-byte block of the
-    // step (x6_at).  A: 128 rows x 4 NPL pieces (2 NPL per thread, fetched into registers one
-    // chunk ahead), B: 64 columns x 4 NPL pieces (NPL per thread, loaded while staging: the B
-    // tile is shared by every row block, so it is usually an L2 hit); consecutive threads read
-    // consecutive pieces.  Rows past n read row n - 1 (their results are not stored)
     constexpr int PW = 4 * NPL;  // pieces per row and step
-    u32x4 ra[2 * NPL];
-    u32x4 rbp[F::PREFETCH_B ? NPL : 1];
-    auto load_b = [&](int k0, u32x4* dst) {
-#pragma unroll
-        for (int u = 0; u < NPL; ++u) {
-            const int q = tid + 256 * u, c = q / PW, w = q % PW;
-            dst[u] = *reinterpret_cast<const u32x4*>(bpl + ((int64_t)(col0 + c) * ksteps + k0 / X6_KC) * (NPL * X6_KC) +
-                                                     8 * w);
-        }
-    };
+    u32x4 ra[2 * NPL], rbp[NPL];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
@@ -738,16 +802,14 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
             const int64_t srow = row0 + r < n ? row0 + r : n - 1;
             ra[u] = *reinterpret_cast<const u32x4*>(apl + (srow * ksteps + k0 / X6_KC) * (NPL * X6_KC) + 8 * w);
         }
-        if constexpr (F::PREFETCH_B) load_b(k0, rbp);
-    };
-    auto stash = [&](int k0) {
-        u32x4 rb[NPL];
-        if constexpr (F::PREFETCH_B) {
 #pragma unroll
-            for (int u = 0; u < NPL; ++u) rb[u] = rbp[u];
-        } else {
-            load_b(k0, rb);
+        for (int u = 0; u < NPL; ++u) {
+            const int q = tid + 256 * u, c = q / PW, w = q % PW;
+            rbp[u] = *reinterpret_cast<const u32x4*>(bpl + ((int64_t)(col0 + c) * ksteps + k0 / X6_KC) * (NPL * X6_KC) +
+                                                     8 * w);
         }
+    };
+    auto stash = [&]() {
 #pragma unroll
         for (int u = 0; u < 2 * NPL; ++u) {
             const int q = tid + 256 * u, r = q / PW, w = q % PW, pl = w / 4, g = w % 4;
@@ -756,64 +818,114 @@ __global__ __launch_bounds__(256, F::MIN_WAVES) void site_gemm_split_kernel(int6
 #pragma unroll
         for (int u = 0; u < NPL; ++u) {
             const int q = tid + 256 * u, c = q / PW, w = q % PW, pl = w / 4, g = w % 4;
-            *reinterpret_cast<u32x4*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rb[u];
+            *reinterpret_cast<u32x4*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8) = rbp[u];
         }
     };
 
-    const int g = lane / 16;
     fetch(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += X6_KC) {
-        stash(k0);
+        stash();
         __syncthreads();
         if (k0 + X6_KC < kend) fetch(k0 + X6_KC);
-        // fragments: lane holds k 8 (lane / 16) .. + 8 of row / column lane % 16
-        V bfr[2][NPL];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int c = wn + 16 * j + lane % 16;
-#pragma unroll
-            for (int pl = 0; pl < NPL; ++pl)
-                bfr[j][pl] = *reinterpret_cast<const V*>(Bs + ((pl * X6_TN + c) * 4 + x6_slot(c, g)) * 8);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = wm + 16 * i + lane % 16;
-            V ar[NP], ai[NP], nai[NP];
-#pragma unroll
-            for (int t = 0; t < NP; ++t) {
-                ar[t] = *reinterpret_cast<const V*>(As + ((t * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
-                ai[t] = *reinterpret_cast<const V*>(As + (((t + NP) * X6_TM + r) * 4 + x6_slot(r, g)) * 8);
-                nai[t] = neg16x8(ai[t]);
-            }
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                acc_re[i][j] = F::prod(ar, &bfr[j][0], acc_re[i][j]);
-                acc_re[i][j] = F::prod(nai, &bfr[j][NP], acc_re[i][j]);
-                acc_im[i][j] = F::prod(ar, &bfr[j][NP], acc_im[i][j]);
-                acc_im[i][j] = F::prod(ai, &bfr[j][0], acc_im[i][j]);
-            }
-        }
+        split_multiply<F>(As, Bs, wm, wn, lane, acc_re, acc_im);
         __syncthreads();
     }
-    // output register r of lane l holds row 4 (l / 16) + r, column l % 16 of each block
+    split_store<F>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T);
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void global_void_t;
+
+// 16 bytes per lane from global memory straight into LDS (global_load_lds_dwordx4): lane l of
+// the wave writes dst + 16 l, so one instruction fills 1 KB of LDS; dst must be wave-uniform.
+__device__ __forceinline__ void lds_dma16(const void* src, void* dst) {
+    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)dst, 16, 0, 0);
+}
+// Waits until at most N of this wave's vector-memory loads (LDS DMA included) are outstanding:
+// s_waitcnt vmcnt(N), with expcnt and lgkmcnt left at their maxima (vmcnt bits 3:0 and 15:14).
+template <int N>
+__device__ __forceinline__ void wait_vm_loads() {
+    static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+    __builtin_amdgcn_s_waitcnt(0x0f70 | (N & 15) | ((N >> 4) << 14));
+}
+
+// Stage of the split-form contraction in LDS: A (the carried vectors' tile) and B (Gamma's).
+template <class F>
+struct SplitStage {
+    __attribute__((aligned(16))) typename F::E a[2 * F::NP * X6_TM * X6_KC];
+    __attribute__((aligned(16))) typename F::E b[2 * F::NP * X6_TN * X6_KC];
+};
+
+// LDS DMA staging.  The two stages are separate LDS objects, so the compiler sees that the reads
+// of one do not depend on the DMA in flight into the other and inserts no wait between them.
+template <class F>
+__global__ __launch_bounds__(256, 1) void site_gemm_split_dma_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                                     const typename F::E* __restrict__ apl,
+                                                                     const float* __restrict__ rowscale,
+                                                                     const typename F::E* __restrict__ bpl,
+                                                                     const float* __restrict__ colscale,
+                                                                     float* __restrict__ T, int kseg) {
+    constexpr int NPL = 2 * F::NP;
+    __shared__ SplitStage<F> st0, st1;
+    int rb, cb;
+    if (!gemm_tile(ncols_pad / X6_TN, (int)((n + X6_TM - 1) / X6_TM), rb, cb)) return;
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int64_t row0 = (int64_t)rb * X6_TM;
+    const int col0 = cb * X6_TN;  // complex column
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 32;
+    const int64_t ksteps = kpad / X6_KC;
+    const int kbeg = (int)blockIdx.y * kseg;
+    const int kend = kbeg + kseg < kpad ? kbeg + kseg : kpad;
+    T += (int64_t)blockIdx.y * n * ncols * 2;
+
+    float4_t acc_re[4][2], acc_im[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) acc_re[i][j] = acc_im[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    // Wave w's DMA instruction u fills the 64 consecutive stage pieces starting at 64 (4 u + w):
+    // plane u / 2, rows 16 (4 (u % 2) + w) to + 15 for A (plane u, columns 16 w to + 15 for B),
+    // four slots each; each lane loads the global piece that belongs in its position -- k group g
+    // of that plane of its row, 16 contiguous bytes of the step's NPL * 64-byte block of the row
+    // (x6_at).  A lane thus needs two row offsets and one column offset; the planes add
+    // constants.  Rows past n read row n - 1 (their results are not stored).
+    constexpr int AU = NPL * X6_TM * 4 / 256, BU = NPL * X6_TN * 4 / 256;  // DMA instructions per wave
+    static_assert(AU == 2 * NPL && BU == NPL, "A: two row groups per plane, B: one column group per plane");
+    int64_t aoff[2];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t srow = row0 + wm + 16 * i + 4 * (lane / 16) + r;
-                const int c = col0 + wn + 16 * j + lane % 16;
-                if (srow < n && c < ncols) {
-                    float re = acc_re[i][j][r], im = acc_im[i][j][r];
-                    if constexpr (F::SCALED) {
-                        const float sc = rowscale[srow] * colscale[c];
-                        re *= sc;
-                        im *= sc;
-                    }
-                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(re, im);
-                }
-            }
+    for (int h = 0; h < 2; ++h) {
+        const int r = (4 * h + wave) * 16 + lane / 4;
+        const int64_t srow = row0 + r < n ? row0 + r : n - 1;
+        aoff[h] = srow * ksteps * (NPL * X6_KC) + x6_slot(r, lane % 4) * 8;  // x6_slot is an involution in g
+    }
+    const int cdma = wave * 16 + lane / 4;
+    const int64_t boff = (int64_t)(col0 + cdma) * ksteps * (NPL * X6_KC) + x6_slot(cdma, lane % 4) * 8;
+    auto issue = [&](int k0, SplitStage<F>& stg) {
+        const int64_t kb = (int64_t)(k0 / X6_KC) * (NPL * X6_KC);
+#pragma unroll
+        for (int u = 0; u < AU; ++u)
+            lds_dma16(apl + aoff[u % 2] + kb + (u / 2) * X6_KC, stg.a + (4 * u + wave) * 64 * 8);
+#pragma unroll
+        for (int u = 0; u < BU; ++u) lds_dma16(bpl + boff + kb + u * X6_KC, stg.b + (4 * u + wave) * 64 * 8);
+    };
+    // One barrier per k step: after it, the stage of this step is complete (every wave waited for
+    // its own DMA into it) and every wave has finished reading the stage of the step before, which
+    // the DMA issued next then overwrites while this step is multiplied.
+    issue(kbeg, st0);
+    for (int k0 = kbeg; k0 < kend; k0 += 2 * X6_KC) {
+        wait_vm_loads<0>();
+        __syncthreads();
+        if (k0 + X6_KC < kend) issue(k0 + X6_KC, st1);
+        split_multiply<F>(st0.a, st0.b, wm, wn, lane, acc_re, acc_im);
+        if (k0 + X6_KC >= kend) break;
+        wait_vm_loads<0>();
+        __syncthreads();
+        if (k0 + 2 * X6_KC < kend) issue(k0 + 2 * X6_KC, st0);
+        split_multiply<F>(st1.a, st1.b, wm, wn, lane, acc_re, acc_im);
+    }
+    split_store<F>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T);
 }
 
 // FP16x3 operand preparation.  Gamma: colscale[c] = max_k max(|Re|, |Im|) of lambda[k] gamma[k][c]
@@ -1669,9 +1781,14 @@ int launch_split_contract(bsmps_ctx* ctx, int64_t n, int64_t ncols, int64_t kpad
         BSMPS_TRY(bsmps_ctx_workspace(ctx, sizeof(float) * (size_t)count * segs, &ws));
         out = static_cast<float*>(ws);
     }
-    hipLaunchKernelGGL(site_gemm_split_kernel<F>, dim3((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs), dim3(256),
-                       0, ctx->stream, n, (int)ncols, (int)kpad, (int)ncols_pad, apl, rowscale, bpl, colscale, out,
-                       segs > 1 ? (int)kSplitKSeg : (int)kpad);
+    const dim3 grid((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs);
+    const int kseg = segs > 1 ? (int)kSplitKSeg : (int)kpad;
+    if constexpr (F::DMA)
+        hipLaunchKernelGGL(site_gemm_split_dma_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
+                           (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg);
+    else
+        hipLaunchKernelGGL(site_gemm_split_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
+                           (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg);
     BSMPS_HIP(hipGetLastError());
     if (segs > 1) {
         hipLaunchKernelGGL(split_k_combine_kernel, dim3((unsigned)std::min<int64_t>((count + 255) / 256, 65536)),
