@@ -555,7 +555,7 @@ def cpu_baseline(inst, kron_out, G, L, pool):
     return out
 
 
-def m144_probe(n_samples=4096):
+def m144_probe(tuning, n_samples=4096):
     """The M = 144, chi = 10^4 workload at full size where one session allows it: the spectra of
     all 143 cuts, the exact hafnian work of every site (pruned plain form), the middle site built
     at full size, and one sampling step through it of ``n_samples`` samples in FP64 and FP16x3.
@@ -575,7 +575,7 @@ def m144_probe(n_samples=4096):
     work = site_hafnian_work_by_site(ko[0], ko[1], inst.d,
                                      site_chi=[int((np.asarray(r) > 1e-10).sum()) for r in ko[0]])
     rate = work[k][1] / t_site
-    probe = probe_sampling(inst, ko, (k, G[k], L[k]), n_samples, {})
+    probe = probe_sampling(inst, ko, (k, G[k], L[k]), n_samples, tuning)
     del G
     total_flops = sum(w[1] for w in work)
     proj_construction = t_kron + total_flops / rate
@@ -932,7 +932,7 @@ def standard_run(args):
     m144 = None
     if not args.no_m144_probe:
         if time.perf_counter() - t_begin < 400:
-            m144 = m144_probe()
+            m144 = m144_probe(tuning)
         else:
             m144 = {"skipped": "the default run had used 400 s before this leg"}
         log(f"M=144 probe done at {time.perf_counter() - t_begin:.0f} s")

@@ -223,9 +223,11 @@ def test_site_contract_bf16x6_matches_matmul():
     ctx = runtime.context()
     rng = np.random.default_rng(9)
     # (200, 2100, ...) has k >= 2048 (128-column tiles) and a last tile past the padded columns;
-    # (96, 9000, ...) has k > 4096, so it runs in k segments added in FP64
+    # (96, 9000, ...) has k > 4096, so it runs in k segments added in FP64; the two-stage k loop
+    # sees one step (k = 20), an odd count (k = 90: 3, k = 130: 5) and even counts
     for n, chi_p, chi, d, spread in ((70, 33, 17, 3, 0), (300, 64, 64, 4, 0), (257, 130, 70, 5, 0),
-                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2), (96, 9000, 20, 3, 0)):
+                                     (129, 100, 30, 6, 6), (200, 2100, 50, 3, 2), (96, 9000, 20, 3, 0),
+                                     (50, 20, 10, 3, 0), (64, 90, 16, 2, 3)):
         scale_a = 10.0 ** rng.uniform(-spread, spread, size=(n, chi_p))
         scale_g = 10.0 ** rng.uniform(-spread, spread, size=(chi_p, chi, d))
         pre = ((rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))) * scale_a).astype(np.complex64)
