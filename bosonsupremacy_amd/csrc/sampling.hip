@@ -606,21 +606,70 @@ __device__ __forceinline__ void split2_fp16(float x, _Float16& h, _Float16& l) {
     l = (_Float16)(x - (float)h);
 }
 
-// parts p of lambda[k] * gamma[k][c] at x6_at(c, k, kpad, 6, p); c < ncols_pad, k < kpad (zero padded)
-__global__ void split_bf16x6_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
-                                    int ncols, int kpad, int ncols_pad, __bf16* __restrict__ planes) {
-    const int64_t total = (int64_t)ncols_pad * kpad;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i / kpad), k = (int)(i % kpad);
-        float2 g = make_float2(0.f, 0.f);
-        if (c < ncols && k < chi_prev) {
-            g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
-            const double l = lam[k];
-            g.x = (float)(g.x * l);
-            g.y = (float)(g.y * l);
+// The NP parts of one value in the format's 16-bit type (exact three-way bf16 split, or the
+// two-way fp16 split of a value already divided by its column scale).
+template <class F>
+__device__ __forceinline__ void split_value(float x, typename F::E (&part)[F::NP]) {
+    if constexpr (F::NP == 3) split3(x, part[0], part[1], part[2]);
+    else split2_fp16(x, part[0], part[1]);
+}
+
+// Gamma's planes for the split-form contraction: parts p of lambda[k] gamma[k][c] (divided by
+// colscale[c] in the scaled format) at x6_at(c, k, kpad, NPL, p) for c < ncols_pad, k < kpad, zero
+// past chi_prev and ncols.  One tile of X6_KC k x 64 columns per iteration: the tile's rows are
+// read 64 consecutive columns at a time (512 contiguous bytes per row), scaled as in an
+// element-wise form, transposed through LDS, and each column's X6_KC k of every plane written as
+// one contiguous NPL * 64-byte block, four threads per column with 16 bytes per plane each.
+// (Thread per element with k fastest, the reads of one wave were 64 rows apart: 0.85 TB/s.)
+template <class F>
+__global__ __launch_bounds__(256) void split_gamma_kernel(const float* __restrict__ gamma,
+                                                          const double* __restrict__ lam,
+                                                          const float* __restrict__ colscale, int chi_prev, int ncols,
+                                                          int kpad, int ncols_pad, typename F::E* __restrict__ planes) {
+    typedef typename F::V V;
+    constexpr int NP = F::NP, NPL = 2 * F::NP;
+    __shared__ float2 tile[X6_KC][64 + 1];
+    const int tid = threadIdx.x;
+    const int nct = ncols_pad / 64;
+    const int64_t ntiles = (int64_t)nct * (kpad / X6_KC);
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int c0 = (int)(t % nct) * 64, k0 = (int)(t / nct) * X6_KC;
+        __syncthreads();  // the previous tile is no longer read
+#pragma unroll
+        for (int i = 0; i < X6_KC * 64 / 256; ++i) {
+            const int e = tid + 256 * i, kk = e / 64, cc = e % 64, k = k0 + kk, c = c0 + cc;
+            float2 g = make_float2(0.f, 0.f);
+            if (c < ncols && k < chi_prev) {
+                g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
+                const double l = lam[k];
+                g.x = (float)(g.x * l);
+                g.y = (float)(g.y * l);
+                if constexpr (F::SCALED) {
+                    const float sc = colscale[c];
+                    g.x = g.x / sc;
+                    g.y = g.y / sc;
+                }
+            }
+            tile[kk][cc] = g;
         }
-        split3(g.x, planes[x6_at(c, k, kpad, 6, 0)], planes[x6_at(c, k, kpad, 6, 1)], planes[x6_at(c, k, kpad, 6, 2)]);
-        split3(g.y, planes[x6_at(c, k, kpad, 6, 3)], planes[x6_at(c, k, kpad, 6, 4)], planes[x6_at(c, k, kpad, 6, 5)]);
+        __syncthreads();
+        const int cc = tid / 4, kq = (tid % 4) * 8;
+        V out[NPL];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float2 g = tile[kq + j][cc];
+            typename F::E re[NP], im[NP];
+            split_value<F>(g.x, re);
+            split_value<F>(g.y, im);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                out[q][j] = re[q];
+                out[NP + q][j] = im[q];
+            }
+        }
+        V* dst = reinterpret_cast<V*>(planes + x6_at(c0 + cc, k0 + kq, kpad, NPL, 0));
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) dst[q * (X6_KC / 8)] = out[q];
     }
 }
 
@@ -930,38 +979,45 @@ __global__ __launch_bounds__(256, 1) void site_gemm_split_dma_kernel(int64_t n, 
 
 // FP16x3 operand preparation.  Gamma: colscale[c] = max_k max(|Re|, |Im|) of lambda[k] gamma[k][c]
 // (1 for an all-zero column), then the two fp16 parts of that entry divided by colscale[c] at
-// x6_at(c, k, kpad, 4, p).  Carried vectors: one wave per row, rowscale[r] = the row's largest
-// component (1 for a zero row), the parts of pre[r][k] / rowscale[r] at x6_at(r, k, kpad, 4, p).
-__global__ void colscale_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam, int chi_prev,
-                                       int ncols, int ncols_pad, float* __restrict__ colscale) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncols_pad; c += gridDim.x * blockDim.x) {
-        float m = 0.f;
-        if (c < ncols)
-            for (int k = 0; k < chi_prev; ++k) {
-                const float2 g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
-                const double l = lam[k];
-                m = fmaxf(m, fmaxf(fabsf((float)(g.x * l)), fabsf((float)(g.y * l))));
-            }
-        colscale[c] = m > 0.f ? m : 1.f;
+// x6_at(c, k, kpad, 4, p) (split_gamma_kernel).  Carried vectors: one wave per row, rowscale[r] =
+// the row's largest component (1 for a zero row), the parts of pre[r][k] / rowscale[r] at
+// x6_at(r, k, kpad, 4, p).
+// colscale (as float bits) = max over k of max(|Re|, |Im|) of lambda[k] gamma[k][c]: blocks of
+// 64 columns x a slice of kslice k, four k lanes per column; the block's maximum goes to the
+// column by atomicMax on the bits (non-negative floats order as their bit patterns, and a
+// maximum does not depend on the order it is taken in).  The caller zeroes the bits first;
+// colscale_finish_kernel then turns them into the scale, 1 for an all-zero column.
+__global__ __launch_bounds__(256) void colmax_fp16x3_kernel(const float* __restrict__ gamma,
+                                                            const double* __restrict__ lam, int chi_prev, int ncols,
+                                                            int kslice, unsigned* __restrict__ bits) {
+    __shared__ float red[4][64];
+    const int cc = threadIdx.x % 64, ks = threadIdx.x / 64;
+    const int c = blockIdx.x * 64 + cc;
+    const int kb = (int)blockIdx.y * kslice, ke = kb + kslice < chi_prev ? kb + kslice : chi_prev;
+    float m = 0.f;
+    if (c < ncols)
+        for (int k = kb + ks; k < ke; k += 4) {
+            const float2 g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
+            const double l = lam[k];
+            m = fmaxf(m, fmaxf(fabsf((float)(g.x * l)), fabsf((float)(g.y * l))));
+        }
+    red[ks][cc] = m;
+    __syncthreads();
+    if (ks == 0 && c < ncols) {
+        m = fmaxf(fmaxf(red[0][cc], red[1][cc]), fmaxf(red[2][cc], red[3][cc]));
+        if (m > 0.f) atomicMax(bits + c, __float_as_uint(m));
     }
 }
 
-__global__ void split_fp16x3_kernel(const float* __restrict__ gamma, const double* __restrict__ lam,
-                                    const float* __restrict__ colscale, int chi_prev, int ncols, int kpad, int ncols_pad,
-                                    _Float16* __restrict__ planes) {
-    const int64_t total = (int64_t)ncols_pad * kpad;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i / kpad), k = (int)(i % kpad);
-        float2 g = make_float2(0.f, 0.f);
-        if (c < ncols && k < chi_prev) {
-            g = reinterpret_cast<const float2*>(gamma)[(int64_t)k * ncols + c];
-            const double l = lam[k];
-            const float sc = colscale[c];
-            g.x = (float)(g.x * l) / sc;
-            g.y = (float)(g.y * l) / sc;
-        }
-        split2_fp16(g.x, planes[x6_at(c, k, kpad, 4, 0)], planes[x6_at(c, k, kpad, 4, 1)]);
-        split2_fp16(g.y, planes[x6_at(c, k, kpad, 4, 2)], planes[x6_at(c, k, kpad, 4, 3)]);
+__global__ void zero_u32_kernel(unsigned* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void colscale_finish_kernel(float* __restrict__ colscale, int ncols_pad) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncols_pad; c += gridDim.x * blockDim.x) {
+        const float m = colscale[c];
+        colscale[c] = m > 0.f ? m : 1.f;
     }
 }
 
@@ -1744,8 +1800,9 @@ extern "C" int bsmps_split_bf16x6(bsmps_ctx* ctx, int chi_prev, int chi, int d, 
         bsmps_set_error("bsmps_split_bf16x6: Gamma too large");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(split_bf16x6_kernel, dim3(grid_for(kpad * ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
-                       lambda_prev, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad, reinterpret_cast<__bf16*>(planes));
+    hipLaunchKernelGGL(split_gamma_kernel<Bf16x6Fmt>, dim3(grid_for(kpad / X6_KC * (ncols_pad / 64), 1)), dim3(256), 0,
+                       ctx->stream, gamma, lambda_prev, nullptr, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad,
+                       reinterpret_cast<__bf16*>(planes));
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
 }
@@ -1845,11 +1902,22 @@ extern "C" int bsmps_split_fp16x3(bsmps_ctx* ctx, int chi_prev, int chi, int d, 
         bsmps_set_error("bsmps_split_fp16x3: Gamma too large");
         return BSMPS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(colscale_fp16x3_kernel, dim3(grid_for(ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
-                       lambda_prev, chi_prev, (int)ncols, (int)ncols_pad, colscale);
+    // column maxima over k slices, enough blocks to fill the GPU (a fill kernel, not a memset, zeroes
+    // the bits, so the call can be captured in a graph)
+    unsigned* bits = reinterpret_cast<unsigned*>(colscale);
+    const int64_t cblocks = (ncols + 63) / 64;
+    const int64_t slices = std::max<int64_t>(1, std::min<int64_t>((chi_prev + 63) / 64, (4096 + cblocks - 1) / cblocks));
+    const int kslice = (int)((chi_prev + slices - 1) / slices);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(ncols_pad, 256)), dim3(256), 0, ctx->stream, bits, ncols_pad);
     BSMPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(split_fp16x3_kernel, dim3(grid_for(kpad * ncols_pad, 256)), dim3(256), 0, ctx->stream, gamma,
-                       lambda_prev, colscale, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad,
+    hipLaunchKernelGGL(colmax_fp16x3_kernel, dim3((unsigned)cblocks, (unsigned)((chi_prev + kslice - 1) / kslice)),
+                       dim3(256), 0, ctx->stream, gamma, lambda_prev, chi_prev, (int)ncols, kslice, bits);
+    BSMPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(colscale_finish_kernel, dim3(grid_for(ncols_pad, 256)), dim3(256), 0, ctx->stream, colscale,
+                       (int)ncols_pad);
+    BSMPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(split_gamma_kernel<Fp16x3Fmt>, dim3(grid_for(kpad / X6_KC * (ncols_pad / 64), 1)), dim3(256), 0,
+                       ctx->stream, gamma, lambda_prev, colscale, chi_prev, (int)ncols, (int)kpad, (int)ncols_pad,
                        reinterpret_cast<_Float16*>(planes));
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
