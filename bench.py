@@ -876,7 +876,8 @@ def standard_run(args):
         s.run(normals, uniforms, timer=ft)
         t_contract = sum(ft.summary().get("site_contract", [0.0]))
         smp, probs = s.run(normals, uniforms, return_probs=True)
-        per, peak = {"fp64": (8, PEAK_FP64_TFLOPS), "fp32": (8, PEAK_FP32_MFMA_TFLOPS),
+        gauss = {p: 6 if tuning.get("contract_" + p) == "gauss" else 8 for p in ("fp64", "fp32")}  # MFMA flops per MAC
+        per, peak = {"fp64": (gauss["fp64"], PEAK_FP64_TFLOPS), "fp32": (gauss["fp32"], PEAK_FP32_MFMA_TFLOPS),
                      "bf16x3": (24, PEAK_BF16_MFMA_TFLOPS), "bf16x6": (48, PEAK_BF16_MFMA_TFLOPS),
                      "fp16x3": (24, PEAK_BF16_MFMA_TFLOPS)}[prec]  # dense FP16 MFMA peak = BF16
         f = {"samples_per_s": n_samples / dt, "contract_ms_per_pass": t_contract * 1e3,

@@ -94,10 +94,10 @@ __global__ void batch_displaces_kernel(int dd, const double* __restrict__ alpha,
 // with A[s][2b] = Re(pre) lambda, A[s][2b+1] = Im(pre) lambda and Bx the 2x2 real block
 // [[Re g, Im g], [-Im g, Re g]] of each complex Gamma entry g -- so C is the complex product
 // stored interleaved, exactly T's layout.  M = n, N = 2 chi d, K = 2 chi_prev; 8 n chi^2 d
-// real flops.  Workgroup tile 128 x 128 x 16 (256 threads, four waves of 64 x 64 = 4 x 4 MFMA
+// real flops.  Workgroup tile 128 x 128 x TKB (256 threads, four waves of 64 x 64 = 4 x 4 MFMA
 // 16x16 blocks); the next K tile is fetched into registers while the current one computes.
 // ---------------------------------------------------------------------------------------
-constexpr int TM = 128, TN = 128, TK = 16;
+constexpr int TM = 128, TN = 128;
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -243,40 +243,60 @@ __global__ __launch_bounds__(256, 2) void site_gemm_kernel(int64_t n, int chi_pr
 // (Gauss's method: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
 // Im = P3 - P1 - P2), 6 n chi_prev chi d real MFMA flops instead of the 8 of the complex
 // product; the extra rounding is a few ulps of |A||B| (the FP64 form stays within 1e-13 of
-// the exact product).  Workgroup tile 128 rows x 64 complex columns x 8 complex k (256
-// threads, four waves of 64 rows x 32 complex columns, 3 x 4 x 2 MFMA 16x16 blocks each); the
-// next K tile is fetched into registers while the current one computes.
+// the exact product).  Four waves of 64 rows x 16 JB complex columns (3 x 4 x JB MFMA 16x16
+// blocks each), WR of them along the rows: workgroup tile 64 WR rows x 64 JB (4 / WR) complex
+// columns x KC complex k; the next K tile is fetched into registers while the current one
+// computes.  FP32: JB = 2, WR = 2, KC = 8.  FP64: JB = 1 (the 24 FP64 accumulators of JB = 2
+// took 256 VGPRs and AGPRs besides, one wave per SIMD), WR = 1 (a square tile: 128 x 32 re-read
+// the carried vectors once per 32 columns) and KC = 16, so a stage's MFMA work still covers the
+// next stage's fetch.
 // Selected per precision by bsmps_set_contract_form (the autotuner times both forms).
 // ---------------------------------------------------------------------------------------
 template <typename E>
-__global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
-                                                        const E* __restrict__ pre, const double* __restrict__ lam,
-                                                        const E* __restrict__ gamma, E* __restrict__ T) {
+struct GaussShape;
+template <>
+struct GaussShape<double> {
+    static constexpr int JB = 1, WR = 1, KC = 16;
+    static constexpr int ROWS = 64 * WR, COLS = 16 * JB * (4 / WR);  // workgroup tile
+};
+template <>
+struct GaussShape<float> {
+    static constexpr int JB = 2, WR = 2, KC = 8;
+    static constexpr int ROWS = 64 * WR, COLS = 16 * JB * (4 / WR);
+};
+
+template <typename E>
+__global__ __launch_bounds__(256, 2) void site_gemm3m_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
+                                                           const E* __restrict__ pre, const double* __restrict__ lam,
+                                                           const E* __restrict__ gamma, E* __restrict__ T) {
     typedef MfmaTraits<E> Tr;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::pair_t E2;
-    constexpr int KC = TK / 2;   // complex k per tile
-    constexpr int NC = TN / 2;   // complex columns per tile
-    __shared__ E Ar[KC][TM + 2], Ai[KC][TM + 2];  // [k][row]
+    constexpr int JB = GaussShape<E>::JB, WC = 4 / GaussShape<E>::WR;  // waves along the columns
+    constexpr int KC = GaussShape<E>::KC;    // complex k per tile
+    constexpr int MR = GaussShape<E>::ROWS;  // rows per tile
+    constexpr int NC = GaussShape<E>::COLS;  // complex columns per tile
+    constexpr int QA = KC * MR / 256, QB = KC * NC / 256;  // complex per thread of the A and B tiles
+    static_assert(QA * 256 == KC * MR && QB * 256 == KC * NC, "tiles split evenly over 256 threads");
+    __shared__ E Ar[KC][MR + 2], Ai[KC][MR + 2];  // [k][row]
     __shared__ E Br[KC][NC + 2], Bi[KC][NC + 2];  // [k][complex col]
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
-    const int64_t row0 = (int64_t)blockIdx.y * TM;
+    const int64_t row0 = (int64_t)blockIdx.y * MR;
     const int64_t ccol0 = (int64_t)blockIdx.x * NC;  // complex column
-    const int wm = (wave / 2) * 64, wn = (wave % 2) * (NC / 2);
+    const int wm = (wave / WC) * 64, wn = (wave % WC) * 16 * JB;
 
     // three real products per 16x16 block (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)
-    acc_t p1[4][2], p2[4][2], p3[4][2];
+    acc_t p1[4][JB], p2[4][JB], p3[4][JB];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < JB; ++j) p1[i][j] = p2[i][j] = p3[i][j] = acc_t{0, 0, 0, 0};
 
-    // per thread: 4 complex of the A tile (128 rows x 8 b) and 2 of the B tile (8 b x 64 c)
-    E2 ra[4], rb[2];
+    E2 ra[QA], rb[QB];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
             const int r = e / KC, bb = e % KC;
             const int64_t srow = row0 + r;
@@ -293,7 +313,7 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
             ra[q] = v;
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
             const int bb = e / NC, cc = e % NC;
             const int b = k0 + bb;
@@ -307,14 +327,14 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q;
             const int r = e / KC, bb = e % KC;
             Ar[bb][r] = ra[q].x;
             Ai[bb][r] = ra[q].y;
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = tid + 256 * q;
             const int bb = e / NC, cc = e % NC;
             Br[bb][cc] = rb[q].x;
@@ -330,7 +350,7 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             // operand layout: lane holds A[lane%16][lane/16] and B[lane/16][lane%16]
-            E ar[4], ai[4], as[4], br[2], bi[2], bs[2];
+            E ar[4], ai[4], as[4], br[JB], bi[JB], bs[JB];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 ar[i] = Ar[kk + lane / 16][wm + i * 16 + lane % 16];
@@ -338,7 +358,7 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
                 as[i] = ar[i] + ai[i];
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+            for (int j = 0; j < JB; ++j) {
                 br[j] = Br[kk + lane / 16][wn + j * 16 + lane % 16];
                 bi[j] = Bi[kk + lane / 16][wn + j * 16 + lane % 16];
                 bs[j] = br[j] + bi[j];
@@ -346,7 +366,7 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
+                for (int j = 0; j < JB; ++j) {
                     p1[i][j] = Tr::mfma(ar[i], br[j], p1[i][j]);
                     p2[i][j] = Tr::mfma(ai[i], bi[j], p2[i][j]);
                     p3[i][j] = Tr::mfma(as[i], bs[j], p3[i][j]);
@@ -358,7 +378,7 @@ __global__ __launch_bounds__(256) void site_gemm3m_kernel(int64_t n, int chi_pre
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t srow = row0 + wm + i * 16 + Tr::row(lane, r);
@@ -1526,6 +1546,13 @@ __global__ void philox_kernel(uint64_t seed, uint32_t stream0, int n_streams, in
     }
 }
 
+// Grid of the Gauss-form contraction: (column tiles, row tiles) of its workgroup tile.
+template <typename E>
+dim3 gauss_grid(int64_t n, int64_t ncols) {
+    return dim3((unsigned)((ncols + GaussShape<E>::COLS - 1) / GaussShape<E>::COLS),
+                (unsigned)((n + GaussShape<E>::ROWS - 1) / GaussShape<E>::ROWS));
+}
+
 int grid_for(int64_t work, int per_block) {
     const int64_t b = (work + per_block - 1) / per_block;
     return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 16));
@@ -1580,12 +1607,13 @@ extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int 
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    if (gx > 0x7fffffff || gy > 65535) {
+    const int64_t gy_gauss = (n + GaussShape<double>::ROWS - 1) / GaussShape<double>::ROWS;
+    if (gx > 0x7fffffff || gy > 65535 || (ctx->contract_gauss[0] && gy_gauss > 65535)) {
         bsmps_set_error("bsmps_site_contract: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
     if (ctx->contract_gauss[0])
-        hipLaunchKernelGGL(site_gemm3m_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+        hipLaunchKernelGGL(site_gemm3m_kernel<double>, gauss_grid<double>(n, ncols), dim3(256), 0, ctx->stream, n,
                            chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     else
         hipLaunchKernelGGL(site_gemm_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
@@ -1670,12 +1698,13 @@ extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, 
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    if (gx > 0x7fffffff || gy > 65535) {
+    const int64_t gy_gauss = (n + GaussShape<float>::ROWS - 1) / GaussShape<float>::ROWS;
+    if (gx > 0x7fffffff || gy > 65535 || (ctx->contract_gauss[1] && gy_gauss > 65535)) {
         bsmps_set_error("bsmps_site_contract_c64: problem too large for one launch");
         return BSMPS_ERR_UNSUPPORTED;
     }
     if (ctx->contract_gauss[1])
-        hipLaunchKernelGGL(site_gemm3m_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
+        hipLaunchKernelGGL(site_gemm3m_kernel<float>, gauss_grid<float>(n, ncols), dim3(256), 0, ctx->stream, n,
                            chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
     else
         hipLaunchKernelGGL(site_gemm_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
