@@ -246,10 +246,10 @@ __global__ __launch_bounds__(256, 2) void site_gemm_kernel(int64_t n, int chi_pr
 // the exact product).  Four waves of 64 rows x 16 JB complex columns (3 x 4 x JB MFMA 16x16
 // blocks each), WR of them along the rows: workgroup tile 64 WR rows x 64 JB (4 / WR) complex
 // columns x KC complex k; the next K tile is fetched into registers while the current one
-// computes.  FP32: JB = 2, WR = 2, KC = 8.  FP64: JB = 1 (the 24 FP64 accumulators of JB = 2
-// took 256 VGPRs and AGPRs besides, one wave per SIMD), WR = 1 (a square tile: 128 x 32 re-read
-// the carried vectors once per 32 columns) and KC = 16, so a stage's MFMA work still covers the
-// next stage's fetch.
+// computes.  FP32: JB = 2, WR = 1 (64 x 128; 128 x 64 was 10% slower), KC = 8.  FP64: JB = 1
+// (the 24 FP64 accumulators of JB = 2 took 256 VGPRs and AGPRs besides, one wave per SIMD),
+// WR = 1 (a square tile: 128 x 32 re-read the carried vectors once per 32 columns) and KC = 16,
+// so a stage's MFMA work still covers the next stage's fetch.
 // Selected per precision by bsmps_set_contract_form (the autotuner times both forms).
 // ---------------------------------------------------------------------------------------
 template <typename E>
@@ -261,7 +261,7 @@ struct GaussShape<double> {
 };
 template <>
 struct GaussShape<float> {
-    static constexpr int JB = 2, WR = 2, KC = 8;
+    static constexpr int JB = 2, WR = 1, KC = 8;
     static constexpr int ROWS = 64 * WR, COLS = 16 * JB * (4 / WR);
 };
 
