@@ -910,13 +910,9 @@ typedef __attribute__((address_space(1))) const void global_void_t;
 __device__ __forceinline__ void lds_dma16(const void* src, void* dst) {
     __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)dst, 16, 0, 0);
 }
-// Waits until at most N of this wave's vector-memory loads (LDS DMA included) are outstanding:
-// s_waitcnt vmcnt(N), with expcnt and lgkmcnt left at their maxima (vmcnt bits 3:0 and 15:14).
-template <int N>
-__device__ __forceinline__ void wait_vm_loads() {
-    static_assert(N >= 0 && N < 64, "vmcnt is six bits");
-    __builtin_amdgcn_s_waitcnt(0x0f70 | (N & 15) | ((N >> 4) << 14));
-}
+// Waits for this wave's outstanding vector-memory loads, LDS DMA included: s_waitcnt vmcnt(0),
+// with expcnt and lgkmcnt left at their maxima.
+__device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 
 // Stage of the split-form contraction in LDS: A (the carried vectors' tile) and B (Gamma's).
 template <class F>
@@ -984,12 +980,12 @@ __global__ __launch_bounds__(256, 1) void site_gemm_split_dma_kernel(int64_t n, 
     // the DMA issued next then overwrites while this step is multiplied.
     issue(kbeg, st0);
     for (int k0 = kbeg; k0 < kend; k0 += 2 * X6_KC) {
-        wait_vm_loads<0>();
+        wait_vm_loads();
         __syncthreads();
         if (k0 + X6_KC < kend) issue(k0 + X6_KC, st1);
         split_multiply<F>(st0.a, st0.b, wm, wn, lane, acc_re, acc_im);
         if (k0 + X6_KC >= kend) break;
-        wait_vm_loads<0>();
+        wait_vm_loads();
         __syncthreads();
         if (k0 + 2 * X6_KC < kend) issue(k0 + 2 * X6_KC, st0);
         split_multiply<F>(st1.a, st1.b, wm, wn, lane, acc_re, acc_im);
