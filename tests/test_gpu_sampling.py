@@ -289,6 +289,59 @@ def _fp16x3_contract(ctx, pre, lam, G):
     return runtime.to_host(T).astype(np.complex128)
 
 
+def test_split_gamma_planes_and_column_scales_exact():
+    """The Gamma planes of both split forms, entry by entry, against a NumPy restatement of the
+    split, on a ragged shape (k and columns padded) with all-zero columns: BF16x6 parts add up to
+    float32(lambda gamma) exactly; FP16x3's column scale is the exact float32 maximum of
+    max(|Re|, |Im|) over k (1 for an all-zero column and for the padded columns) and its two parts
+    are the float16 roundings h = fp16(x / scale), l = fp16(x / scale - h)."""
+    from bosonsupremacy_amd import runtime
+    ctx = runtime.context()
+    rng = np.random.default_rng(21)
+    chi_p, chi, d = 45, 14, 5                        # kpad 64, ncols 70 -> ncols_pad 128
+    ncols, kpad, cpad = chi * d, 64, 128
+    G = ((rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d)))
+         * 10.0 ** rng.uniform(-3, 3, size=(chi_p, chi, d))).astype(np.complex64)
+    G[:, 3, 2] = 0                                   # column 17 all zero
+    G[:, 9, 0] = 0                                   # column 45 all zero
+    lam = rng.uniform(0.01, 1, size=chi_p)
+    # the kernels' scaling: float32(float64(g) * lambda)
+    x = G.reshape(chi_p, ncols)
+    xr = (x.real.astype(np.float64) * lam[:, None]).astype(np.float32)
+    xi = (x.imag.astype(np.float64) * lam[:, None]).astype(np.float32)
+    G_d, lam_d = runtime.to_device(G), runtime.to_device(lam)
+    print(f"split planes chi_p={chi_p} ncols={ncols}", flush=True)
+
+    planes = torch.empty((ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.bfloat16,
+                         device=ctx.device)
+    ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G_d), runtime.ptr(lam_d), runtime.ptr(planes))
+    P = planes.view(cpad, kpad // 32, 6, 32).permute(2, 0, 1, 3).reshape(6, cpad, kpad).double().cpu().numpy()
+    re, im = P[0] + P[1] + P[2], P[3] + P[4] + P[5]   # [column][k]
+    assert np.array_equal(re[:ncols, :chi_p], xr.T.astype(np.float64))
+    assert np.array_equal(im[:ncols, :chi_p], xi.T.astype(np.float64))
+    assert not P[:, ncols:, :].any() and not P[:, :, chi_p:].any()
+
+    planes = torch.empty((ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.float16,
+                         device=ctx.device)
+    colscale = torch.full((cpad,), -7.0, dtype=torch.float32, device=ctx.device)  # overwritten entirely
+    ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(G_d), runtime.ptr(lam_d), runtime.ptr(planes),
+             runtime.ptr(colscale))
+    m = np.maximum(np.abs(xr), np.abs(xi)).max(axis=0)
+    want_sc = np.ones(cpad, dtype=np.float32)
+    want_sc[:ncols] = np.where(m > 0, m, np.float32(1))
+    got_sc = runtime.to_host(colscale)
+    assert np.array_equal(got_sc, want_sc)
+    assert got_sc[17] == 1 and got_sc[45] == 1
+    P = planes.view(cpad, kpad // 32, 4, 32).permute(2, 0, 1, 3).reshape(4, cpad, kpad).cpu().numpy()
+    for part, v in ((0, xr), (2, xi)):
+        y = (v / want_sc[None, :ncols]).astype(np.float32).T          # [column][k], float32 division
+        h = y.astype(np.float16)
+        lo = (y - h.astype(np.float32)).astype(np.float16)
+        assert np.array_equal(P[part][:ncols, :chi_p], h)
+        assert np.array_equal(P[part + 1][:ncols, :chi_p], lo)
+    assert not P[:, ncols:, :].any() and not P[:, :, chi_p:].any()
+
+
 def test_site_contract_fp16x3_matches_matmul():
     """FP16x3 form: every entry within K * 2^-19 * (row scale) * (column scale) of the exact
     product (per product the error is ~2^-21 of the scaled operands), on ragged shapes and on
