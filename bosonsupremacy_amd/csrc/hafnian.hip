@@ -828,7 +828,8 @@ __global__ __launch_bounds__(kWave) void haf_subtree_kernel(BfsData bd, const in
         for (int K = 2 * K0; K < J; ++K)
 #pragma unroll
             for (int t = 0; t < D; ++t)
-                if (t < live_degree(M, J, K)) P[tri(J, K)][t] = bfs_ld(bd, po + tri(J - ob, K - ob) * pcap + t, h);
+                if (t < live_degree(M, J, K))  // read once per launch: streamed past the L2
+                    P[tri(J, K)][t] = ld_cd_nt(bd.data, (int64_t)(po + tri(J - ob, K - ob) * pcap + t) * bd.stride + h);
     cd g[M + 1];
 #pragma unroll
     for (int t = 0; t <= M; ++t) g[t] = t <= D ? bfs_ld(bd, fr.w + t, h) : czero();
