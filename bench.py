@@ -122,8 +122,10 @@ def device_copy_gbs(nbytes=4 << 30, reps=5):
 
 def probe_sampling(inst, ko, probe, n, tuning):
     """Time one site's sampling step at full size: the contraction of n carried vectors with
-    a real Gamma_k (FP64, FP32, BF16x6 and FP16x3 forms) and the displacement + draw, then
-    extrapolate the chain's samples/s as n / ((M - 1) x per-site time)."""
+    a real Gamma_k in the FP64, FP32, BF16x6 and FP16x3 forms, prepared and contracted exactly as
+    the sampler does (split into its two photon-number parity blocks when the tensor has them), and
+    the displacement + draw; then extrapolate the chain's samples/s as n / ((M - 1) x per-site
+    time)."""
     import torch
     from bosonsupremacy_amd import runtime, sampling
     ctx = runtime.context()
@@ -142,7 +144,7 @@ def probe_sampling(inst, ko, probe, n, tuning):
     rng = torch.Generator(device=dev).manual_seed(0)
     pre = torch.randn((n, chi_p), dtype=torch.complex128, device=dev, generator=rng)
     pre /= torch.linalg.vector_norm(pre, dim=1, keepdim=True)
-    T = torch.empty((n, chi, d), dtype=torch.complex128, device=dev)
+    pre = pre.reshape(-1).contiguous()
 
     def timed(fn, reps=2):
         fn()
@@ -155,41 +157,35 @@ def probe_sampling(inst, ko, probe, n, tuning):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / reps / 1e3
 
-    t64 = timed(lambda: ctx.call("bsmps_site_contract", n, chi_p, chi, d, runtime.ptr(pre), runtime.ptr(lam_prev),
-                                 runtime.ptr(G), runtime.ptr(T)))
-    G32 = G.to(torch.complex64)
-    del G
-    pre32 = pre.to(torch.complex64)
-    T32 = torch.empty((n, chi, d), dtype=torch.complex64, device=dev)
-    t32 = timed(lambda: ctx.call("bsmps_site_contract_c64", n, chi_p, chi, d, runtime.ptr(pre32),
-                                 runtime.ptr(lam_prev), runtime.ptr(G32), runtime.ptr(T32)))
-    # BF16x6: Gamma split once per site (not timed, as in the sampler); the carried vectors are
-    # split once per step (timed with the contraction)
-    planes = torch.empty((ctx.lib.bsmps_bf16x6_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.bfloat16, device=dev)
-    ctx.call("bsmps_split_bf16x6", chi_p, chi, d, runtime.ptr(G32), runtime.ptr(lam_prev), runtime.ptr(planes))
-    rows = torch.empty((ctx.lib.bsmps_bf16x6_row_plane_bytes(n, chi_p) // 2,), dtype=torch.bfloat16, device=dev)
-
-    def x6():
-        ctx.call("bsmps_split_rows_bf16x6", n, chi_p, runtime.ptr(pre32), runtime.ptr(rows))
-        ctx.call("bsmps_site_contract_bf16x6", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(planes),
-                 runtime.ptr(T32))
-    t6 = timed(x6)
-    del planes, rows
-    # FP16x3: the same with two scaled fp16 parts
-    planes = torch.empty((ctx.lib.bsmps_fp16x3_plane_bytes(chi_p, chi, d) // 2,), dtype=torch.float16, device=dev)
-    colscale = torch.empty(((chi * d + 63) // 64 * 64,), dtype=torch.float32, device=dev)
-    ctx.call("bsmps_split_fp16x3", chi_p, chi, d, runtime.ptr(G32), runtime.ptr(lam_prev), runtime.ptr(planes),
-             runtime.ptr(colscale))
-    del G32
-    rows = torch.empty((ctx.lib.bsmps_fp16x3_row_plane_bytes(n, chi_p) // 2,), dtype=torch.float16, device=dev)
-    rowscale = torch.empty((n,), dtype=torch.float32, device=dev)
-
-    def h3():
-        ctx.call("bsmps_split_rows_fp16x3", n, chi_p, runtime.ptr(pre32), runtime.ptr(rows), runtime.ptr(rowscale))
-        ctx.call("bsmps_site_contract_fp16x3", n, chi_p, chi, d, runtime.ptr(rows), runtime.ptr(rowscale),
-                 runtime.ptr(planes), runtime.ptr(colscale), runtime.ptr(T32))
-    t3 = timed(h3)
-    del planes, rows
+    forms = {key: tuning.get(key, "blocks") for key in ("contract_fp64", "contract_fp32")}
+    full_macs = n * chi_p * chi * d
+    out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_forms": forms}
+    per_mac = {"fp64": 6.0 if forms["contract_fp64"] == "gauss" else 8.0,
+               "fp32": 6.0 if forms["contract_fp32"] == "gauss" else 8.0, "bf16x6": 48.0, "fp16x3": 24.0}
+    peak = {"fp64": PEAK_FP64_TFLOPS, "fp32": PEAK_FP32_MFMA_TFLOPS, "bf16x6": PEAK_BF16_MFMA_TFLOPS,
+            "fp16x3": PEAK_BF16_MFMA_TFLOPS}  # dense FP16 MFMA peak = BF16
+    T = None
+    times = {}
+    for prec in ("fp64", "fp32", "bf16x6", "fp16x3"):
+        ctype = torch.complex128 if prec == "fp64" else torch.complex64
+        # the site's operands are prepared once per site (not timed, as in the sampler); the
+        # carried vectors' split and the parity gathers and scatters are timed with the products
+        f = sampling._prepare_site(ctx, prec, k, G if prec == "fp64" else G.to(torch.complex64), chi_p, chi, lam_prev)
+        work = sampling._Work(ctx, prec, n, chi_p, chi * d, ctype)
+        pre_f = pre if prec == "fp64" else pre.to(torch.complex64)
+        T = torch.empty((n * chi * d,), dtype=ctype, device=dev)
+        t = timed(lambda: sampling.contract_site(ctx, prec, f, n, pre_f, lam_prev, T, work))
+        macs = sum(n * b.rows * b.cols for b in f.blocks) if f.blocks is not None else full_macs
+        times[prec] = t
+        out[f"contract_{prec}_s"] = t
+        out[f"contract_{prec}_parity_split"] = f.blocks is not None
+        out[f"contract_{prec}_executed_macs"] = macs
+        out[f"contract_{prec}_effective_complex_gemm_tflops"] = 8.0 * full_macs / t / 1e12
+        out[f"contract_{prec}_mfma_utilisation"] = per_mac[prec] * macs / t / 1e12 / peak[prec]
+        del f, work, pre_f
+        if prec == "fp64":
+            T64 = T
+        torch.cuda.empty_cache()
     W, x = sampling.displacement_basis(inst.dd)
     Wd, xd = runtime.to_device(W), runtime.to_device(x)
     alpha = (torch.randn((n, inst.M), dtype=torch.complex128, device=dev, generator=rng) * 0.3).contiguous()
@@ -197,26 +193,13 @@ def probe_sampling(inst, ko, probe, n, tuning):
     smp = torch.empty((n, 1), dtype=torch.int32, device=dev)
     nxt = torch.empty((n, chi), dtype=torch.complex128, device=dev)
     dwork = torch.empty((2 * n * inst.dd * d,), dtype=torch.float64, device=dev)
-    ts = timed(lambda: ctx.call("bsmps_sample_site", n, chi, d, inst.dd, runtime.ptr(T), chi * d,
+    ts = timed(lambda: ctx.call("bsmps_sample_site", n, chi, d, inst.dd, runtime.ptr(T64), chi * d,
                                 runtime.ptr(alpha), inst.M, runtime.ptr(Wd), runtime.ptr(xd), runtime.ptr(dwork),
                                 runtime.ptr(lam),
                                 runtime.ptr(u), runtime.ptr(smp), 1, runtime.ptr(nxt), None))
-    forms = {k: tuning.get(k, "blocks") for k in ("contract_fp64", "contract_fp32")}
-    f64 = (6.0 if forms["contract_fp64"] == "gauss" else 8.0) * n * chi_p * chi * d  # MFMA flops executed
-    f32 = (6.0 if forms["contract_fp32"] == "gauss" else 8.0) * n * chi_p * chi * d
-    useful = 8.0 * n * chi_p * chi * d  # complex-GEMM flops
-    out = {"site": k, "n": n, "gamma_shape": [chi_p, chi, d], "contract_fp64_s": t64, "contract_fp32_s": t32,
-           "contract_bf16x6_s": t6, "contract_fp16x3_s": t3, "sample_site_s": ts, "contract_forms": forms,
-           "contract_fp64_tflops": f64 / t64 / 1e12, "contract_fp32_tflops": f32 / t32 / 1e12,
-           "contract_bf16x6_complex_gemm_tflops": useful / t6 / 1e12,
-           "contract_bf16x6_mfma_utilisation": 6 * useful / t6 / 1e12 / PEAK_BF16_MFMA_TFLOPS,
-           "contract_fp16x3_complex_gemm_tflops": useful / t3 / 1e12,
-           "contract_fp16x3_mfma_utilisation": 3 * useful / t3 / 1e12 / PEAK_BF16_MFMA_TFLOPS,
-           "chain_samples_per_s_fp64": n / ((inst.M - 1) * (t64 + ts)),
-           "chain_samples_per_s_fp32": n / ((inst.M - 1) * (t32 + ts)),
-           "chain_samples_per_s_bf16x6": n / ((inst.M - 1) * (t6 + ts)),
-           "chain_samples_per_s_fp16x3": n / ((inst.M - 1) * (t3 + ts))}
-    log(f"sampling probe at site {k}: {out}")
+    out["sample_site_s"] = ts
+    for prec in ("fp64", "fp32", "bf16x6", "fp16x3"):
+        out[f"chain_samples_per_s_{prec}"] = n / ((inst.M - 1) * (times[prec] + ts))
     return out
 
 
@@ -686,8 +669,22 @@ MARGINAL_BAR = 1e-6
 
 
 def form_flops(s, n):
-    """Complex multiply-adds of one pass's site contractions over the live ranks of sampler s."""
+    """Complex multiply-adds of one pass's site contractions over the live ranks of sampler s (the
+    full products, zero parity blocks included)."""
     return sum(n * s.rows[i] * s.chis[i] * s.ds[i] for i in range(1, s.M))
+
+
+def executed_macs(s, n):
+    """Complex multiply-adds the contractions of one pass of sampler s execute: the two parity
+    blocks of every site that splits, the full live block of one that does not."""
+    total = 0
+    for i in range(1, s.M):
+        f = s.forms[i]
+        if f.blocks is not None:
+            total += sum(n * b.rows * b.cols for b in f.blocks)
+        else:
+            total += n * s.rows[i] * s.chis[i] * s.ds[i]
+    return total
 
 
 def chain_run(args):
@@ -814,8 +811,10 @@ def standard_run(args):
         tot = mean * per_step
         e = {"launches_per_step": per_step, "mean_ms": mean * 1e3, "total_ms_per_step": tot * 1e3}
         if name == "site_contract":
+            # the full products over the live ranks; the parity split executes the fraction below
             e["tflops"] = 8 * cmacs / tot / 1e12
             e["flops_per_step"] = 8.0 * cmacs
+            e["executed_macs_fraction"] = executed_macs(s0, n_samples) / cmacs
         if name == "gamma_site":
             e.update({"hafnians_per_step": n_haf, "algorithmic_flops_per_step": haf_alg,
                       "executed_flops_per_step": haf_exec, "tflops": haf_alg / tot / 1e12,
@@ -880,9 +879,11 @@ def standard_run(args):
         per, peak = {"fp64": (gauss["fp64"], PEAK_FP64_TFLOPS), "fp32": (gauss["fp32"], PEAK_FP32_MFMA_TFLOPS),
                      "bf16x3": (24, PEAK_BF16_MFMA_TFLOPS), "bf16x6": (48, PEAK_BF16_MFMA_TFLOPS),
                      "fp16x3": (24, PEAK_BF16_MFMA_TFLOPS)}[prec]  # dense FP16 MFMA peak = BF16
+        emacs = executed_macs(s, n_samples)
         f = {"samples_per_s": n_samples / dt, "contract_ms_per_pass": t_contract * 1e3,
-             "contract_complex_gemm_tflops": 8 * cmacs / t_contract / 1e12,
-             "contract_mfma_utilisation": per * cmacs / t_contract / 1e12 / peak}
+             "contract_effective_complex_gemm_tflops": 8 * cmacs / t_contract / 1e12,
+             "contract_executed_macs_fraction": emacs / cmacs,
+             "contract_mfma_utilisation": per * emacs / t_contract / 1e12 / peak}
         if prec == "fp64":
             e1 = metrics.rb_first_order(probs)
             ref = {"e1": e1, "e2": metrics.rb_second_order(smp, probs), "samples": smp,

@@ -109,10 +109,111 @@ def _as_tensor(g):
     return g if isinstance(g, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(g))
 
 
+# Photon-number parity of the site tensors.  Every entry of Gamma_i is a hafnian of the zero-mean
+# pure state (MPS_cpu.py:117-187), whose matrix size is the photon total of (left label, j, right
+# label); an odd total gives a zero entry.  So Gamma_i[a, b, j] = 0 unless p(a) = p(b) xor (j mod 2)
+# for the label parities p, and the contraction splits into two independent products of half the
+# size: the left labels with p = 0 against the columns (b, j) with p(b) xor j = 0, and the labels
+# with p = 1 against the other columns.  The parities are read off Gamma itself (a two-colouring of
+# its non-zero pattern, checked entry by entry), so a tensor with this structure -- built here or
+# read from the reference's files -- takes the split, and one without it keeps the full product.
+PARITY_MIN_RANK = 64   # smallest left rank (and column count / 2) worth splitting
+
+
+def parity_blocks(g):
+    """For a device tensor g [rows][cols][d]: ((rows0, cols0), (rows1, cols1)), index tensors of
+    the two parity blocks into g's rows and into its flattened cols * d columns, or None when g's
+    non-zero entries do not split by parity, a block would be empty, or g is too small to gain."""
+    rows, cols, d = (int(x) for x in g.shape)
+    if rows < PARITY_MIN_RANK or cols * d < 2 * PARITY_MIN_RANK or d < 2:
+        return None
+    nz = g != 0
+    E = nz[:, :, 0::2].any(dim=2)   # (a, b) joined through an even j
+    O = nz[:, :, 1::2].any(dim=2)   # ... through an odd j
+    if bool((E & O).any()):
+        return None
+    dev = g.device
+    pa = torch.full((rows,), -1, dtype=torch.int8, device=dev)
+    pb = torch.full((cols,), -1, dtype=torch.int8, device=dev)
+    row_any = (E | O).any(dim=1)
+    while True:
+        seeds = torch.nonzero((pa < 0) & row_any)
+        if len(seeds) == 0:
+            break
+        pa[int(seeds[0])] = 0                 # a new connected part: its first label even
+        for _ in range(rows + cols):          # alternate until the part is coloured
+            a0, a1 = (pa == 0)[:, None], (pa == 1)[:, None]
+            b_even = ((E & a0) | (O & a1)).any(dim=0)
+            b_odd = ((E & a1) | (O & a0)).any(dim=0)
+            b0, b1 = (pb == 0)[None, :], (pb == 1)[None, :]
+            a_even = ((E & b0) | (O & b1)).any(dim=1)
+            a_odd = ((E & b1) | (O & b0)).any(dim=1)
+            if bool((b_even & b_odd).any()) or bool((a_even & a_odd).any()):
+                return None
+            new_pb = torch.where(b_even, 0, torch.where(b_odd, 1, pb.long())).to(torch.int8)
+            new_pa = torch.where(a_even, 0, torch.where(a_odd, 1, pa.long())).to(torch.int8)
+            if torch.equal(new_pa, pa) and torch.equal(new_pb, pb):
+                break
+            pa, pb = new_pa, new_pb
+    pa = torch.where(pa < 0, 0, pa.long()).bool()   # labels without a non-zero entry: either block
+    pb = torch.where(pb < 0, 0, pb.long()).bool()
+    same = pa[:, None] == pb[None, :]
+    if bool((E & ~same).any()) or bool((O & same).any()):
+        return None
+    pc = (pb[:, None] ^ (torch.arange(d, device=dev) % 2 == 1)[None, :]).reshape(-1)
+    blocks = []
+    for cls in (False, True):
+        ia = torch.nonzero(pa == cls).squeeze(1)
+        ic = torch.nonzero(pc == cls).squeeze(1)
+        if len(ia) == 0 or len(ic) == 0:
+            return None
+        blocks.append((ia, ic))
+    return tuple(blocks)
+
+
 class _SiteForm:
     """Device data of one site in one precision: the live block Gamma_i[:rows, :cols, :d] and, for
-    the split forms, its pre-split operand planes (built once per site, reused by every batch)."""
-    __slots__ = ("rows", "cols", "d", "gamma", "split")
+    the split forms, its pre-split operand planes (built once per site, reused by every batch).
+    When the site splits by photon-number parity, ``blocks`` holds the two blocks instead (each a
+    _SiteBlock) and ``gamma`` is dropped past site 0."""
+    __slots__ = ("rows", "cols", "d", "gamma", "split", "blocks")
+
+
+class _SiteBlock:
+    """One parity block of a site: its left labels ``ia`` and columns ``ic`` (index tensors), the
+    block of Gamma as a [len(ia), len(ic), 1] tensor, Lambda_{i-1} of its labels and, for the
+    split forms, its operand planes."""
+    __slots__ = ("ia", "ic", "rows", "cols", "gamma", "lam", "split")
+
+
+def _split_operand(ctx, precision, gamma, rows, cols, d, lam_prev):
+    """The reduced forms' once-per-site operand of gamma [rows][cols][d] (device, complex64 for the
+    split forms): hi / lo bf16 planes (bf16x3), six bf16 planes of Lambda_{i-1} Gamma (bf16x6), or
+    the scaled fp16 planes and their column scales (fp16x3); None for fp64 / fp32."""
+    dev = ctx.device
+    if precision == "bf16x3":
+        # hi / lo bf16 planes of Gamma_i
+        cnt = 2 * gamma.numel()
+        hi = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+        lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
+        ctx.call("bsmps_split_bf16", runtime.ptr(gamma), cnt, runtime.ptr(hi), runtime.ptr(lo))
+        return hi, lo
+    if precision == "bf16x6":
+        # six bf16 planes of Lambda_{i-1} Gamma_i (FP32-exact split)
+        nbytes = ctx.lib.bsmps_bf16x6_plane_bytes(rows, cols, d)
+        planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
+        ctx.call("bsmps_split_bf16x6", rows, cols, d, runtime.ptr(gamma), runtime.ptr(lam_prev), runtime.ptr(planes))
+        return planes
+    if precision == "fp16x3":
+        # two fp16 planes per part of Lambda_{i-1} Gamma_i, columns scaled to a largest component of
+        # 1 (the scales are multiplied back by the contraction)
+        nbytes = ctx.lib.bsmps_fp16x3_plane_bytes(rows, cols, d)
+        planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
+        colscale = torch.empty((-(-(cols * d) // 64) * 64,), dtype=torch.float32, device=dev)
+        ctx.call("bsmps_split_fp16x3", rows, cols, d, runtime.ptr(gamma), runtime.ptr(lam_prev), runtime.ptr(planes),
+                 runtime.ptr(colscale))
+        return planes, colscale
+    return None
 
 
 def _prepare_site(ctx, precision, i, g, rows, cols, lam_prev):
@@ -126,33 +227,79 @@ def _prepare_site(ctx, precision, i, g, rows, cols, lam_prev):
     g = _as_tensor(g)
     f.rows, f.cols, f.d = rows, cols, int(g.shape[2])
     f.gamma = g[:rows, :cols].to(device=dev, dtype=ctype).contiguous()
-    f.split = None
+    f.split = f.blocks = None
     if i == 0:
         return f
-    if precision == "bf16x3":
-        # hi / lo bf16 planes of Gamma_i
-        cnt = 2 * f.gamma.numel()
-        hi = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
-        lo = torch.empty((cnt,), dtype=torch.bfloat16, device=dev)
-        ctx.call("bsmps_split_bf16", runtime.ptr(f.gamma), cnt, runtime.ptr(hi), runtime.ptr(lo))
-        f.split = (hi, lo)
-    elif precision == "bf16x6":
-        # six bf16 planes of Lambda_{i-1} Gamma_i (FP32-exact split)
-        nbytes = ctx.lib.bsmps_bf16x6_plane_bytes(rows, cols, f.d)
-        planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
-        ctx.call("bsmps_split_bf16x6", rows, cols, f.d, runtime.ptr(f.gamma), runtime.ptr(lam_prev),
-                 runtime.ptr(planes))
-        f.split = planes
-    elif precision == "fp16x3":
-        # two fp16 planes per part of Lambda_{i-1} Gamma_i, columns scaled to a largest component of
-        # 1 (the scales are multiplied back by the contraction)
-        nbytes = ctx.lib.bsmps_fp16x3_plane_bytes(rows, cols, f.d)
-        planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
-        colscale = torch.empty((-(-(cols * f.d) // 64) * 64,), dtype=torch.float32, device=dev)
-        ctx.call("bsmps_split_fp16x3", rows, cols, f.d, runtime.ptr(f.gamma), runtime.ptr(lam_prev),
-                 runtime.ptr(planes), runtime.ptr(colscale))
-        f.split = (planes, colscale)
+    pb = parity_blocks(f.gamma)
+    if pb is None:
+        f.split = _split_operand(ctx, precision, f.gamma, rows, cols, f.d, lam_prev)
+        return f
+    flat = f.gamma.view(rows, cols * f.d)
+    f.blocks = []
+    for ia, ic in pb:
+        b = _SiteBlock()
+        b.ia, b.ic, b.rows, b.cols = ia, ic, len(ia), len(ic)
+        b.gamma = flat.index_select(0, ia).index_select(1, ic).contiguous().view(b.rows, b.cols, 1)
+        b.lam = lam_prev.index_select(0, ia).contiguous()
+        b.split = _split_operand(ctx, precision, b.gamma, b.rows, b.cols, 1, b.lam)
+        f.blocks.append(b)
+    f.gamma = None
     return f
+
+
+def contract_site(ctx, precision, f, nc, pre, lam_prev, T, work):
+    """T [nc][cols * d] = (pre [nc][rows] Lambda_{i-1}) Gamma_i for the site form f (every sample's
+    carried vector against the site tensor), in one product or, when f splits by parity, in two
+    half-size ones: each block's labels gathered from the carried vectors, its product into
+    ``work.T`` and its columns scattered into T.  ``work`` holds the reduced forms' row planes
+    (``pre_planes``, ``rowscale``) and the block output buffer ``T`` (at least nc * cols * d)."""
+    if f.blocks is None:
+        _contract(ctx, precision, nc, f.rows, f.cols, f.d, runtime.ptr(pre), lam_prev, f.gamma, f.split, T, work)
+        return
+    Tv = T[:nc * f.cols * f.d].view(nc, f.cols * f.d)
+    pre_v = pre[:nc * f.rows].view(nc, f.rows)
+    for b in f.blocks:
+        pre_b = pre_v.index_select(1, b.ia).contiguous()
+        Tb = work.T[:nc * b.cols]
+        _contract(ctx, precision, nc, b.rows, b.cols, 1, runtime.ptr(pre_b), b.lam, b.gamma, b.split, Tb, work)
+        Tv.index_copy_(1, b.ic, Tb.view(nc, b.cols))
+
+
+def _contract(ctx, precision, nc, rows, cols, d, pre_in, lam_prev, gamma, split, T, work):
+    T = runtime.ptr(T)
+    if precision == "fp64":
+        ctx.call("bsmps_site_contract", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(gamma), T)
+    elif precision == "fp32":
+        ctx.call("bsmps_site_contract_c64", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(gamma), T)
+    elif precision == "fp16x3":
+        planes, colscale = split
+        ctx.call("bsmps_split_rows_fp16x3", nc, rows, pre_in, runtime.ptr(work.pre_planes),
+                 runtime.ptr(work.rowscale))
+        ctx.call("bsmps_site_contract_fp16x3", nc, rows, cols, d, runtime.ptr(work.pre_planes),
+                 runtime.ptr(work.rowscale), runtime.ptr(planes), runtime.ptr(colscale), T)
+    elif precision == "bf16x6":
+        ctx.call("bsmps_split_rows_bf16x6", nc, rows, pre_in, runtime.ptr(work.pre_planes))
+        ctx.call("bsmps_site_contract_bf16x6", nc, rows, cols, d, runtime.ptr(work.pre_planes), runtime.ptr(split), T)
+    else:
+        hi, lo = split
+        ctx.call("bsmps_site_contract_bf16x3", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(hi),
+                 runtime.ptr(lo), T)
+
+
+class _Work:
+    """Scratch of the contraction: row planes of the reduced forms and a block output buffer."""
+
+    def __init__(self, ctx, precision, chunk, chi_max, cd_max, ctype):
+        dev = ctx.device
+        self.pre_planes = self.rowscale = None
+        if precision == "bf16x6":
+            nbytes = ctx.lib.bsmps_bf16x6_row_plane_bytes(chunk, chi_max)
+            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
+        if precision == "fp16x3":
+            nbytes = ctx.lib.bsmps_fp16x3_row_plane_bytes(chunk, chi_max)
+            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
+            self.rowscale = torch.empty((chunk,), dtype=torch.float32, device=dev)
+        self.T = torch.empty((chunk * cd_max,), dtype=ctype, device=dev)
 
 
 def _offset(t, elems):
@@ -189,14 +336,7 @@ class _Walk:
         self.T = torch.empty((self.chunk * cd_max,), dtype=ctype, device=dev)
         self.probs = torch.empty((M, n, dd), dtype=torch.float64, device=dev) if return_probs else None
         self.dwork = torch.empty((2 * self.chunk * dd * d_max,), dtype=torch.float64, device=dev)  # D_s
-        self.pre_planes = self.rowscale = None
-        if precision == "bf16x6":
-            nbytes = ctx.lib.bsmps_bf16x6_row_plane_bytes(self.chunk, chi_max)
-            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.bfloat16, device=dev)
-        if precision == "fp16x3":
-            nbytes = ctx.lib.bsmps_fp16x3_row_plane_bytes(self.chunk, chi_max)
-            self.pre_planes = torch.empty((nbytes // 2,), dtype=torch.float16, device=dev)
-            self.rowscale = torch.empty((self.chunk,), dtype=torch.float32, device=dev)
+        self.work = _Work(ctx, precision, self.chunk, chi_max, cd_max, ctype)
 
     def site(self, i, f, lam_prev, lam, uniforms_row, timer):
         """Advance every sample through site i (form ``f``); ``lam_prev`` / ``lam`` are
@@ -211,9 +351,8 @@ class _Walk:
             if i == 0:
                 src, stride = f.gamma, 0   # Gamma_0[0, a, j] is the site-0 tensor of every sample
             else:
-                pre_in = _offset(src_buf, c0 * f.rows)
                 with timer("site_contract"):
-                    self._contract(nc, f, pre_in, lam_prev)
+                    contract_site(ctx, prec, f, nc, src_buf[c0 * f.rows:], lam_prev, self.T, self.work)
                 src, stride = self.T, f.cols * f.d
             with timer("sample_site"):
                 ctx.call(draw, nc, f.cols, f.d, dd, runtime.ptr(src), stride, _offset(self.alpha, c0 * M + i), M,
@@ -223,30 +362,6 @@ class _Walk:
                          _offset(self.probs[i], c0 * dd) if self.probs is not None else None)
         if len(self.pre) == 2:
             self.cur = 1 - self.cur
-
-    def _contract(self, nc, f, pre_in, lam_prev):
-        ctx, prec = self.ctx, self.precision
-        rows, cols, d = f.rows, f.cols, f.d
-        T = runtime.ptr(self.T)
-        if prec == "fp64":
-            ctx.call("bsmps_site_contract", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(f.gamma), T)
-        elif prec == "fp32":
-            ctx.call("bsmps_site_contract_c64", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev),
-                     runtime.ptr(f.gamma), T)
-        elif prec == "fp16x3":
-            planes, colscale = f.split
-            ctx.call("bsmps_split_rows_fp16x3", nc, rows, pre_in, runtime.ptr(self.pre_planes),
-                     runtime.ptr(self.rowscale))
-            ctx.call("bsmps_site_contract_fp16x3", nc, rows, cols, d, runtime.ptr(self.pre_planes),
-                     runtime.ptr(self.rowscale), runtime.ptr(planes), runtime.ptr(colscale), T)
-        elif prec == "bf16x6":
-            ctx.call("bsmps_split_rows_bf16x6", nc, rows, pre_in, runtime.ptr(self.pre_planes))
-            ctx.call("bsmps_site_contract_bf16x6", nc, rows, cols, d, runtime.ptr(self.pre_planes),
-                     runtime.ptr(f.split), T)
-        else:
-            hi, lo = f.split
-            ctx.call("bsmps_site_contract_bf16x3", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev),
-                     runtime.ptr(hi), runtime.ptr(lo), T)
 
 
 class Sampler:

@@ -289,6 +289,64 @@ def _fp16x3_contract(ctx, pre, lam, G):
     return runtime.to_host(T).astype(np.complex128)
 
 
+def _parity_gamma(rng, rows, cols, d, parity_a, parity_b):
+    """Random complex Gamma [rows][cols][d] that is zero wherever p(a) != p(b) xor (j mod 2)."""
+    g = rng.normal(size=(rows, cols, d)) + 1j * rng.normal(size=(rows, cols, d))
+    j = np.arange(d)
+    keep = (parity_a[:, None, None] ^ parity_b[None, :, None] ^ (j[None, None, :] % 2)) == 0
+    return np.where(keep, g, 0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16x3"])
+def test_parity_split_contraction_matches_full(prec):
+    """Site tensors whose entries vanish by photon-number parity are contracted as two half-size
+    products (gathered labels, scattered columns); the result equals the full product (FP64 within
+    1e-13 of its scale, FP16x3 within its FP32-level bound), tensors without the structure are
+    not split, and a tensor of two disconnected label groups still is."""
+    from bosonsupremacy_amd import runtime, sampling
+    ctx = runtime.context()
+    rng = np.random.default_rng(31)
+    rows, cols, d, n = 150, 90, 5, 300
+    pa, pb = rng.integers(0, 2, size=rows), rng.integers(0, 2, size=cols)
+    G = _parity_gamma(rng, rows, cols, d, pa, pb)
+    G[7] = 0                                               # a label with no non-zero entry
+    lam = rng.uniform(0.1, 1, size=rows)
+    pre = rng.normal(size=(n, rows)) + 1j * rng.normal(size=(n, rows))
+    want = ((pre * lam) @ G.reshape(rows, cols * d))
+    ctype = torch.complex128 if prec == "fp64" else torch.complex64
+    print(f"parity split {prec} rows={rows} cols={cols} d={d} n={n}", flush=True)
+    lam_d = runtime.to_device(lam)
+    f = sampling._prepare_site(ctx, prec, 1, torch.from_numpy(G), rows, cols, lam_d)
+    assert f.blocks is not None and f.gamma is None
+    (ia0, ic0), (ia1, ic1) = [(b.ia.cpu().numpy(), b.ic.cpu().numpy()) for b in f.blocks]
+    assert sorted(np.concatenate([ia0, ia1]).tolist()) == list(range(rows))
+    assert sorted(np.concatenate([ic0, ic1]).tolist()) == list(range(cols * d))
+    work = sampling._Work(ctx, prec, n, rows, cols * d, ctype)
+    pre_d = torch.from_numpy(pre).to(device=ctx.device, dtype=ctype).reshape(-1).contiguous()
+    T = torch.empty((n * cols * d,), dtype=ctype, device=ctx.device)
+    sampling.contract_site(ctx, prec, f, n, pre_d, lam_d, T, work)
+    got = runtime.to_host(T).reshape(n, cols * d).astype(np.complex128)
+    if prec == "fp64":
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+    else:
+        bound = (np.abs(pre * lam) @ np.abs(G.reshape(rows, cols * d)))
+        assert (np.abs(got - want) / bound).max() < 1e-5
+    # one entry against the structure: no split
+    G2 = G.copy()
+    a_bad = int(np.flatnonzero(pa == 0)[0]); b_bad = int(np.flatnonzero(pb == 0)[0])
+    G2[a_bad, b_bad, 1] = 1.0
+    assert sampling.parity_blocks(torch.from_numpy(G2).to(ctx.device)) is None
+    # two disconnected label groups (block-diagonal in the labels): still split, checked entry-wise
+    G3 = np.zeros_like(G)
+    G3[:75, :45] = G[:75, :45]
+    G3[75:, 45:] = G[75:, 45:]
+    blocks = sampling.parity_blocks(torch.from_numpy(G3).to(ctx.device))
+    assert blocks is not None
+    flat = G3.reshape(rows, cols * d)
+    (ia0, ic0), (ia1, ic1) = [(a.cpu().numpy(), c.cpu().numpy()) for a, c in blocks]
+    assert not flat[np.ix_(ia0, ic1)].any() and not flat[np.ix_(ia1, ic0)].any()
+
+
 def test_split_gamma_planes_and_column_scales_exact():
     """The Gamma planes of both split forms, entry by entry, against a NumPy restatement of the
     split, on a ragged shape (k and columns padded) with all-zero columns: BF16x6 parts add up to
