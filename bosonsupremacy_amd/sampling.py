@@ -117,49 +117,58 @@ def _as_tensor(g):
 # with p = 1 against the other columns.  The parities are read off Gamma itself (a two-colouring of
 # its non-zero pattern, checked entry by entry), so a tensor with this structure -- built here or
 # read from the reference's files -- takes the split, and one without it keeps the full product.
-PARITY_MIN_RANK = 64   # smallest left rank (and column count / 2) worth splitting
+# The split costs a search of Gamma's non-zero pattern and two gathers once per site, and a gather
+# and a scatter per batch of samples: below about 2^24 live entries (rows x cols x d) a site's
+# product saves less than that, so smaller sites keep the full product.
+PARITY_MIN_ENTRIES = 1 << 24
 
 
 def parity_blocks(g):
     """For a device tensor g [rows][cols][d]: ((rows0, cols0), (rows1, cols1)), index tensors of
     the two parity blocks into g's rows and into its flattened cols * d columns, or None when g's
-    non-zero entries do not split by parity, a block would be empty, or g is too small to gain."""
+    non-zero entries do not split by parity, a block would be empty, or g is below
+    PARITY_MIN_ENTRIES."""
     rows, cols, d = (int(x) for x in g.shape)
-    if rows < PARITY_MIN_RANK or cols * d < 2 * PARITY_MIN_RANK or d < 2:
+    if d < 2 or rows * cols * d < PARITY_MIN_ENTRIES:
         return None
     nz = g != 0
     E = nz[:, :, 0::2].any(dim=2)   # (a, b) joined through an even j
     O = nz[:, :, 1::2].any(dim=2)   # ... through an odd j
-    if bool((E & O).any()):
-        return None
+    del nz
     dev = g.device
     pa = torch.full((rows,), -1, dtype=torch.int8, device=dev)
     pb = torch.full((cols,), -1, dtype=torch.int8, device=dev)
     row_any = (E | O).any(dim=1)
-    while True:
-        seeds = torch.nonzero((pa < 0) & row_any)
-        if len(seeds) == 0:
+
+    def propagate(pa, pb):
+        a0, a1 = (pa == 0)[:, None], (pa == 1)[:, None]
+        b_even = ((E & a0) | (O & a1)).any(dim=0)
+        b_odd = ((E & a1) | (O & a0)).any(dim=0)
+        pb = torch.where(b_even & (pb < 0), 0, torch.where(b_odd & (pb < 0), 1, pb.long())).to(torch.int8)
+        b0, b1 = (pb == 0)[None, :], (pb == 1)[None, :]
+        a_even = ((E & b0) | (O & b1)).any(dim=1)
+        a_odd = ((E & b1) | (O & b0)).any(dim=1)
+        pa = torch.where(a_even & (pa < 0), 0, torch.where(a_odd & (pa < 0), 1, pa.long())).to(torch.int8)
+        return pa, pb
+
+    # colour one connected part at a time from a seed label (even); a dense pattern is coloured in
+    # one or two rounds, so rounds run in fours between the checks for uncoloured labels
+    for _ in range(rows):
+        unseen = torch.nonzero((pa < 0) & row_any)
+        if len(unseen) == 0:
             break
-        pa[int(seeds[0])] = 0                 # a new connected part: its first label even
-        for _ in range(rows + cols):          # alternate until the part is coloured
-            a0, a1 = (pa == 0)[:, None], (pa == 1)[:, None]
-            b_even = ((E & a0) | (O & a1)).any(dim=0)
-            b_odd = ((E & a1) | (O & a0)).any(dim=0)
-            b0, b1 = (pb == 0)[None, :], (pb == 1)[None, :]
-            a_even = ((E & b0) | (O & b1)).any(dim=1)
-            a_odd = ((E & b1) | (O & b0)).any(dim=1)
-            if bool((b_even & b_odd).any()) or bool((a_even & a_odd).any()):
-                return None
-            new_pb = torch.where(b_even, 0, torch.where(b_odd, 1, pb.long())).to(torch.int8)
-            new_pa = torch.where(a_even, 0, torch.where(a_odd, 1, pa.long())).to(torch.int8)
-            if torch.equal(new_pa, pa) and torch.equal(new_pb, pb):
+        pa[int(unseen[0])] = 0
+        for _ in range(rows + cols):
+            before = int((pa >= 0).sum()) + int((pb >= 0).sum())
+            for _ in range(4):
+                pa, pb = propagate(pa, pb)
+            if int((pa >= 0).sum()) + int((pb >= 0).sum()) == before:
                 break
-            pa, pb = new_pa, new_pb
     pa = torch.where(pa < 0, 0, pa.long()).bool()   # labels without a non-zero entry: either block
     pb = torch.where(pb < 0, 0, pb.long()).bool()
     same = pa[:, None] == pb[None, :]
-    if bool((E & ~same).any()) or bool((O & same).any()):
-        return None
+    if bool(((E & ~same) | (O & same)).any()):    # every non-zero entry checked (an (a, b) pair
+        return None                                 # joined through both parities fails here too)
     pc = (pb[:, None] ^ (torch.arange(d, device=dev) % 2 == 1)[None, :]).reshape(-1)
     blocks = []
     for cls in (False, True):

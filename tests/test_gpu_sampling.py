@@ -67,8 +67,12 @@ def _mps_m8():
     return inst, G, L
 
 
-def test_sampling_m8_matches_oracle(contract_form):
+@pytest.mark.parametrize("parity", ["full", "split"])
+def test_sampling_m8_matches_oracle(contract_form, parity, monkeypatch):
+    """GPU sampler vs the oracle on the M = 8 workload, with every site contracted in one product
+    and with every site that splits by photon-number parity contracted as its two blocks."""
     from bosonsupremacy_amd import runtime, sampling
+    monkeypatch.setattr(sampling, "PARITY_MIN_ENTRIES", 0 if parity == "split" else 1 << 62)
     inst, G, L = _mps_m8()
     sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
     n = 3000
@@ -76,6 +80,7 @@ def test_sampling_m8_matches_oracle(contract_form):
     normals = rng.normal(size=(2 * inst.M, n))
     uniforms = rng.uniform(size=(inst.M, n))
     s = sampling.Sampler(G, L, sqrtW, inst.dd)
+    assert any(f.blocks is not None for f in s.forms) == (parity == "split")
     got, probs = s.run(runtime.to_device(normals), runtime.to_device(uniforms), return_probs=True)
     got, probs = runtime.to_host(got), runtime.to_host(probs)
     oG = [runtime.to_host(g) for g in G]
@@ -298,12 +303,13 @@ def _parity_gamma(rng, rows, cols, d, parity_a, parity_b):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp16x3"])
-def test_parity_split_contraction_matches_full(prec):
+def test_parity_split_contraction_matches_full(prec, monkeypatch):
     """Site tensors whose entries vanish by photon-number parity are contracted as two half-size
     products (gathered labels, scattered columns); the result equals the full product (FP64 within
     1e-13 of its scale, FP16x3 within its FP32-level bound), tensors without the structure are
     not split, and a tensor of two disconnected label groups still is."""
     from bosonsupremacy_amd import runtime, sampling
+    monkeypatch.setattr(sampling, "PARITY_MIN_ENTRIES", 0)   # a small tensor, split regardless
     ctx = runtime.context()
     rng = np.random.default_rng(31)
     rows, cols, d, n = 150, 90, 5, 300
