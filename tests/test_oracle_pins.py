@@ -9,8 +9,12 @@ against closed forms of Gaussian states, not against themselves:
 * each conditional distribution the sampler draws from equals that of the displaced pure
   state D(alpha)|psi> given the sample's own displacement alpha and the photon numbers drawn
   before it;
+* every site tensor's entry Gamma_k[a, b, j] vanishes unless the photon total of the labels and
+  j is even (hafnians of odd size are zero), and the two-colouring the library splits each site
+  by (sampling.parity_blocks, from the non-zero pattern alone) recovers the photon-number parity
+  of every Schmidt label from the kron tables;
 * mistakes planted in the oracle (a dropped sqrt(j!), a transposed site tensor, a transposed
-  displacement matrix) fail these checks.
+  displacement matrix, a site tensor shifted by one photon) fail these checks.
 
 The instance is small and weakly squeezed so that the truncations (photon cutoff d, Schmidt
 weights below 1e-10) drop ~1e-6 of the weight; the tolerances sit an order above that and
@@ -132,3 +136,68 @@ def test_planted_mistakes_fail(pinned, monkeypatch):
         mp.setattr(O, "batch_displaces", lambda N, al: np.swapaxes(real_displaces(N, al), -1, -2))
         err, _ = sampler_error(inst, G, L, B)
     assert err > 100 * PROB_TOL
+
+
+def _parities(kron, k, live_only):
+    """Photon-number parities of the labels of cut k (kron_cpu.py tables).  A site's columns are
+    the labels of its own cut kept where the Schmidt weight exceeds 1e-10 (MPS_cpu.py:470-477);
+    its rows are every label of the previous cut (MPS_cpu.py:463-468, no filter)."""
+    res, num = kron[0][k], kron[1][k]
+    num = num.reshape(num.shape[0], -1)
+    if live_only:
+        num = num[res > 1e-10]
+    return num.sum(axis=1).astype(np.int64) % 2
+
+
+def _site_parities(kron, k):
+    prow = _parities(kron, k - 1, False)
+    pcol = _parities(kron, k, True) if k < M - 1 else np.zeros(1, dtype=np.int64)
+    return prow, pcol
+
+
+def _colouring_error(G, kron, monkeypatch):
+    """For each site k >= 1: None if parity_blocks finds no split, else the number of live labels
+    (rows and columns with a non-zero entry) whose colour differs from their photon-number parity
+    after the one flip the colouring is defined up to."""
+    import torch
+    from bosonsupremacy_amd import sampling
+    monkeypatch.setattr(sampling, "PARITY_MIN_ENTRIES", 0)
+    out = []
+    for k in range(1, M):
+        prow, pcol = _site_parities(kron, k)
+        rows, cols, d = len(prow), len(pcol), G[k].shape[2]
+        g = np.ascontiguousarray(G[k][:rows, :cols])
+        blocks = sampling.parity_blocks(torch.from_numpy(g))
+        if blocks is None:
+            out.append(None)
+            continue
+        (_, _), (ia1, ic1) = blocks
+        crow = np.zeros(rows, dtype=np.int64)
+        crow[ia1.numpy()] = 1
+        ccol = np.zeros(cols * d, dtype=np.int64)
+        ccol[ic1.numpy()] = 1
+        phys_col = (pcol[:, None] ^ (np.arange(d)[None, :] % 2)).reshape(-1)
+        nz = g != 0
+        live_r, live_c = nz.any(axis=(1, 2)), nz.any(axis=0).reshape(-1)
+        flip = int(crow[np.flatnonzero(live_r)[0]] ^ prow[np.flatnonzero(live_r)[0]])
+        out.append(int(((crow ^ prow ^ flip) != 0)[live_r].sum() + ((ccol ^ phys_col ^ flip) != 0)[live_c].sum()))
+    return out
+
+
+def test_site_tensors_split_by_photon_number_parity(pinned, monkeypatch):
+    """Gamma_k[a, b, j] != 0 only if p(a) + p(b) + j is even, p = the label's photon-number parity,
+    and the library's colouring of the non-zero pattern is exactly that parity (the instance's
+    patterns are connected, so one flip at most); a site tensor shifted by one photon along j
+    breaks the structure and is not split."""
+    inst, kron, G, L, B, detQ = pinned
+    for k in range(1, M):
+        prow, pcol = _site_parities(kron, k)
+        g = G[k][:len(prow), :len(pcol)]
+        odd = (prow[:, None, None] + pcol[None, :, None] + np.arange(g.shape[2])[None, None, :]) % 2 == 1
+        assert not g[odd].any() and g[~odd].any()
+        assert not G[k][len(prow):].any() and not G[k][:, len(pcol):].any()   # past the labels: zero
+    assert _colouring_error(G, kron, monkeypatch) == [0] * (M - 1)
+    # planted: Gamma_k[:, :, j] <- Gamma_k[:, :, j - 1] at the middle sites (photon count off by one)
+    G4 = [np.roll(g, 1, axis=2) if k == 2 else g for k, g in enumerate(G)]
+    errs = _colouring_error(G4, kron, monkeypatch)
+    assert errs[1] is None or errs[1] > 0
