@@ -61,6 +61,45 @@ def test_sampler_accepts_host_site_tensors():
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp16x3"])
+def test_site_major_walks_with_parity_split(precision, monkeypatch):
+    """Every site past the first split by photon-number parity (the size threshold forced to 0):
+    the site-major walk in chunks, with host tensors, equals the all-resident sampler bit for bit,
+    the construct-and-sample chain draws the same samples, and the split sampler's FP64
+    probabilities equal the unsplit sampler's to rounding."""
+    from bosonsupremacy_amd import chain, runtime, sampling
+    inst, G, L, kron_out = _m8()
+    sqrtW = sampling.thermal_sqrtW(inst.cov, inst.sq_cov)
+    n = 1000
+    normals, uniforms = sampling.random_inputs(inst.M, n, seed=5)
+    s0, p0 = sampling.Sampler(G, L, sqrtW, inst.dd, precision=precision).run(normals, uniforms, return_probs=True)
+    monkeypatch.setattr(sampling, "PARITY_MIN_ENTRIES", 0)
+    real, split = sampling.parity_blocks, []
+
+    def counted(g):
+        b = real(g)
+        split.append(b is not None)
+        return b
+    monkeypatch.setattr(sampling, "parity_blocks", counted)
+    print(f"parity split walks {precision} M={inst.M} n={n}", flush=True)
+    s, p = sampling.Sampler(G, L, sqrtW, inst.dd, precision=precision).run(normals, uniforms, return_probs=True)
+    assert len(split) == inst.M - 1 and all(split)
+    if precision == "fp64":
+        assert torch.equal(s, s0)
+        assert (p - p0).abs().max().item() < 1e-13
+    w = sampling.SiteSampler(inst.M, inst.dd, sqrtW, normals, uniforms, inst.chi, inst.d, precision=precision,
+                             chunk=300)
+    for k in range(inst.M):
+        w.push(k, runtime.to_host(G[k]), L[k] if k < inst.M - 1 else None)
+    assert torch.equal(w.samples, s) and torch.equal(w.probs, p)
+    assert len(split) == 2 * (inst.M - 1) and all(split)
+    if precision == "fp64":
+        c = chain.Chain(inst.sq_cov, inst.cov, inst.d, inst.chi, inst.dd, n, seed=5, chunk=400, kron=kron_out)
+        c.run()
+        assert same_bits(c.samples(0), runtime.to_host(s).astype(np.int64))
+        assert all(split)
+
+
 def test_chain_matches_sampler_and_resumes(tmp_path):
     """Construct-and-sample over the M = 32 workload: the FP64 walk equals the all-resident
     sampler's, the FP16x3 replay stays on its paths within 1e-6, and a chain stopped after site
