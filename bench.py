@@ -318,7 +318,7 @@ def partial_run(args):
     print(json.dumps(out))
 
 
-def survey_hafnian_flops(size):
+def reference_hafnian_flops(size):
     """Algorithmic FP64 flops of one hafnian of the given size in the reference recursion
     (MPS_cpu.py:117-187) with every loop run in full: F(2n) = 8 C(n) + 8 2^n - 2 with
     C(n) = sum_{l<n} 2^l n(n+1)/2 (1 + (2n-2l-2)(2n-2l-3)); F(4) = 22 (three products and two
@@ -335,7 +335,7 @@ def survey_hafnian_flops(size):
 def site_hafnian_work(kres, knum, d, err_tol=1e-10, pruned=False):
     """(hafnian count, executed flops, algorithmic flops) of all site tensors, from the label
     photon counts.  Executed flops skip structurally zero and dead terms (hafnian_flops);
-    algorithmic flops are the reference recursion's full loops (survey_hafnian_flops).
+    algorithmic flops are the reference recursion's full loops (reference_hafnian_flops).
     ``pruned``: only the kept labels of cut k-1 enter site k (mps.build_mps(pruned=True))."""
     M = len(kres) + 1
     total_n, total_f, total_a = 0, 0.0, 0.0
@@ -361,7 +361,7 @@ def site_hafnian_work(kres, knum, d, err_tol=1e-10, pruned=False):
                         continue
                     cnt = int(ca[p]) * int(cb[q])
                     if n // 2 not in fl:
-                        fl[n // 2] = (hafnian_flops(n // 2), survey_hafnian_flops(n))
+                        fl[n // 2] = (hafnian_flops(n // 2), reference_hafnian_flops(n))
                     total_n += cnt
                     total_f += cnt * fl[n // 2][0]
                     total_a += cnt * fl[n // 2][1]
