@@ -96,6 +96,14 @@ _PROTOTYPES = {
     "bsmps_split_rows_fp16x3": ([c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "bsmps_site_contract_fp16x3": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p], c_int),
+    "bsmps_site_contract_cols": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_int64, c_void_p], c_int),
+    "bsmps_site_contract_c64_cols": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_int64, c_void_p], c_int),
+    "bsmps_site_contract_bf16x6_cols": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                         c_int64, c_void_p], c_int),
+    "bsmps_site_contract_fp16x3_cols": ([c_void_p, c_int64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_int64, c_void_p], c_int),
     "bsmps_philox": ([c_void_p, c_uint64, c_uint32, c_int, c_int64, c_int64, c_int, c_void_p], c_int),
     "bsmps_mlp_forward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 10, c_int),
     "bsmps_mlp_backward": ([c_void_p, c_int, c_int, c_int, c_int] + [c_void_p] * 13, c_int),

@@ -125,10 +125,19 @@ struct MfmaTraits<float> {
     __device__ static int row(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
-template <typename E>
+// Where the contraction kernels put output column c (complex) of sample row s: T[s][c] in a row of
+// ncols when MAPPED is false, else T[s * ld + map[c]] (rows of ld complex values; an entry outside
+// [0, ld) is not written).  The photon-number parity split contracts each parity block of a site
+// separately and scatters its columns this way.
+struct ColMap {
+    int64_t ld;
+    const int* map;
+};
+
+template <typename E, bool MAPPED = false>
 __global__ __launch_bounds__(256, 2) void site_gemm_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
                                                         const E* __restrict__ pre, const double* __restrict__ lam,
-                                                        const E* __restrict__ gamma, E* __restrict__ T) {
+                                                        const E* __restrict__ gamma, E* __restrict__ T, ColMap om) {
     typedef MfmaTraits<E> Tr;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::pair_t E2;
@@ -233,7 +242,14 @@ __global__ __launch_bounds__(256, 2) void site_gemm_kernel(int64_t n, int chi_pr
             for (int r = 0; r < 4; ++r) {
                 const int64_t srow = row0 + wm + i * 16 + Tr::row(lane, r);
                 const int64_t c = col0 + wn + j * 16 + lane % 16;
-                if (srow < n && c < Nreal) T[srow * Nreal + c] = acc[i][j][r];
+                if (srow < n && c < Nreal) {
+                    if constexpr (MAPPED) {
+                        const int64_t oc = om.map[c >> 1];
+                        if (oc >= 0 && oc < om.ld) T[(srow * om.ld + oc) * 2 + (c & 1)] = acc[i][j][r];
+                    } else {
+                        T[srow * Nreal + c] = acc[i][j][r];
+                    }
+                }
             }
 }
 
@@ -265,10 +281,10 @@ struct GaussShape<float> {
     static constexpr int ROWS = 64 * WR, COLS = 16 * JB * (4 / WR);
 };
 
-template <typename E>
+template <typename E, bool MAPPED = false>
 __global__ __launch_bounds__(256, 2) void site_gemm3m_kernel(int64_t n, int chi_prev, int ncols /* complex cols = chi*d */,
                                                            const E* __restrict__ pre, const double* __restrict__ lam,
-                                                           const E* __restrict__ gamma, E* __restrict__ T) {
+                                                           const E* __restrict__ gamma, E* __restrict__ T, ColMap om) {
     typedef MfmaTraits<E> Tr;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::pair_t E2;
@@ -387,7 +403,12 @@ __global__ __launch_bounds__(256, 2) void site_gemm3m_kernel(int64_t n, int chi_
                     E2 v;
                     v.x = p1[i][j][r] - p2[i][j][r];
                     v.y = (p3[i][j][r] - p1[i][j][r]) - p2[i][j][r];
-                    reinterpret_cast<E2*>(T)[srow * ncols + c] = v;
+                    if constexpr (MAPPED) {
+                        const int64_t oc = om.map[c];
+                        if (oc >= 0 && oc < om.ld) reinterpret_cast<E2*>(T)[srow * om.ld + oc] = v;
+                    } else {
+                        reinterpret_cast<E2*>(T)[srow * ncols + c] = v;
+                    }
                 }
             }
 }
@@ -743,6 +764,19 @@ __global__ __launch_bounds__(256) void split_k_combine_kernel(const float* __res
     }
 }
 
+// The same sum written through a column map (ColMap): partial element i is part i % 2 of complex
+// column (i / 2) % ncols of row (i / 2) / ncols.
+__global__ __launch_bounds__(256) void split_k_combine_cols_kernel(const float* __restrict__ P, int64_t n, int ncols,
+                                                                   int segs, float* __restrict__ T, ColMap om) {
+    const int64_t count = n * ncols * 2;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) {
+        double acc = 0.0;
+        for (int s = 0; s < segs; ++s) acc += (double)P[(int64_t)s * count + i];
+        const int64_t e = i >> 1, row = e / ncols, oc = om.map[e - row * ncols];
+        if (oc >= 0 && oc < om.ld) T[(row * om.ld + oc) * 2 + (i & 1)] = (float)acc;
+    }
+}
+
 // Split-form contraction kernels: F::NP parts per value, 2 F::NP planes per operand (parts of
 // the real part, then of the imaginary part).  With F::SCALED the planes hold values divided by a
 // per-row (A) and per-column (B) scale, which the epilogue multiplies back.
@@ -803,11 +837,11 @@ __device__ __forceinline__ void split_multiply(const typename F::E* As, const ty
 }
 
 // Epilogue: output register r of lane l holds row 4 (l / 16) + r, column l % 16 of each block.
-template <class F>
+template <class F, bool MAPPED>
 __device__ __forceinline__ void split_store(const float4_t (&acc_re)[4][2], const float4_t (&acc_im)[4][2],
                                             int64_t n, int ncols, int64_t row0, int col0, int wm, int wn, int lane,
                                             const float* __restrict__ rowscale, const float* __restrict__ colscale,
-                                            float* __restrict__ T) {
+                                            float* __restrict__ T, ColMap om) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -823,7 +857,12 @@ __device__ __forceinline__ void split_store(const float4_t (&acc_re)[4][2], cons
                         re *= sc;
                         im *= sc;
                     }
-                    reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(re, im);
+                    if constexpr (MAPPED) {
+                        const int64_t oc = om.map[c];
+                        if (oc >= 0 && oc < om.ld) reinterpret_cast<float2*>(T)[srow * om.ld + oc] = make_float2(re, im);
+                    } else {
+                        reinterpret_cast<float2*>(T)[srow * ncols + c] = make_float2(re, im);
+                    }
                 }
             }
 }
@@ -833,13 +872,13 @@ __device__ __forceinline__ void split_store(const float4_t (&acc_re)[4][2], cons
 // step ahead), B: 64 columns x 4 NPL pieces (NPL per thread, fetched one step ahead as well);
 // consecutive threads read consecutive pieces.  Rows past n read row n - 1 (their results are
 // not stored).
-template <class F>
-__global__ __launch_bounds__(256, 2) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
-                                                                 const typename F::E* __restrict__ apl,
-                                                                 const float* __restrict__ rowscale,
-                                                                 const typename F::E* __restrict__ bpl,
-                                                                 const float* __restrict__ colscale,
-                                                                 float* __restrict__ T, int kseg) {
+template <class F, bool MAPPED>
+__device__ __forceinline__ void split_gemm_registers(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                     const typename F::E* __restrict__ apl,
+                                                     const float* __restrict__ rowscale,
+                                                     const typename F::E* __restrict__ bpl,
+                                                     const float* __restrict__ colscale, float* __restrict__ T,
+                                                     int kseg, ColMap om) {
     typedef typename F::E E;
     constexpr int NPL = 2 * F::NP;
     __shared__ __attribute__((aligned(16))) E As[NPL * X6_TM * X6_KC];
@@ -899,7 +938,28 @@ __global__ __launch_bounds__(256, 2) void site_gemm_split_kernel(int64_t n, int 
         split_multiply<F>(As, Bs, wm, wn, lane, acc_re, acc_im);
         __syncthreads();
     }
-    split_store<F>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T);
+    split_store<F, MAPPED>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T, om);
+}
+
+// The register form as its own kernels, plain and with a column map (separate signatures keep
+// the plain kernel's code independent of the map).
+template <class F>
+__global__ __launch_bounds__(256, 2) void site_gemm_split_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                                 const typename F::E* __restrict__ apl,
+                                                                 const float* __restrict__ rowscale,
+                                                                 const typename F::E* __restrict__ bpl,
+                                                                 const float* __restrict__ colscale,
+                                                                 float* __restrict__ T, int kseg) {
+    split_gemm_registers<F, false>(n, ncols, kpad, ncols_pad, apl, rowscale, bpl, colscale, T, kseg, ColMap{0, nullptr});
+}
+template <class F>
+__global__ __launch_bounds__(256, 2) void site_gemm_split_cols_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
+                                                                      const typename F::E* __restrict__ apl,
+                                                                      const float* __restrict__ rowscale,
+                                                                      const typename F::E* __restrict__ bpl,
+                                                                      const float* __restrict__ colscale,
+                                                                      float* __restrict__ T, int kseg, ColMap om) {
+    split_gemm_registers<F, true>(n, ncols, kpad, ncols_pad, apl, rowscale, bpl, colscale, T, kseg, om);
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -923,13 +983,13 @@ struct SplitStage {
 
 // LDS DMA staging.  The two stages are separate LDS objects, so the compiler sees that the reads
 // of one do not depend on the DMA in flight into the other and inserts no wait between them.
-template <class F>
+template <class F, bool MAPPED = false>
 __global__ __launch_bounds__(256, 1) void site_gemm_split_dma_kernel(int64_t n, int ncols, int kpad, int ncols_pad,
                                                                      const typename F::E* __restrict__ apl,
                                                                      const float* __restrict__ rowscale,
                                                                      const typename F::E* __restrict__ bpl,
                                                                      const float* __restrict__ colscale,
-                                                                     float* __restrict__ T, int kseg) {
+                                                                     float* __restrict__ T, int kseg, ColMap om) {
     constexpr int NPL = 2 * F::NP;
     __shared__ SplitStage<F> st0, st1;
     int rb, cb;
@@ -990,7 +1050,7 @@ __global__ __launch_bounds__(256, 1) void site_gemm_split_dma_kernel(int64_t n, 
         if (k0 + 2 * X6_KC < kend) issue(k0 + 2 * X6_KC, st0);
         split_multiply<F>(st1.a, st1.b, wm, wn, lane, acc_re, acc_im);
     }
-    split_store<F>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T);
+    split_store<F, MAPPED>(acc_re, acc_im, n, ncols, row0, col0, wm, wn, lane, rowscale, colscale, T, om);
 }
 
 // FP16x3 operand preparation.  Gamma: colscale[c] = max_k max(|Re|, |Im|) of lambda[k] gamma[k][c]
@@ -1590,32 +1650,61 @@ extern "C" int bsmps_batch_displaces(bsmps_ctx* ctx, int dd, const double* alpha
     return BSMPS_OK;
 }
 
-extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const double* pre,
-                                   const double* lambda_prev, const double* gamma, double* T) {
-    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
-        bsmps_set_error("bsmps_site_contract: invalid arguments");
+// FP64 (E = double) and FP32 (E = float) contraction in the form the context selects (Gauss or
+// blocks), stored plainly (om null) or through a column map.
+template <typename E>
+static int site_contract_mfma(bsmps_ctx* ctx, const char* name, int64_t n, int chi_prev, int chi, int d, const E* pre,
+                              const double* lambda_prev, const E* gamma, E* T, const ColMap* om) {
+    constexpr int form = sizeof(E) == 8 ? 0 : 1;
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0 || (om && om->ld <= 0)) {
+        bsmps_set_error("%s: invalid arguments", name);
         return BSMPS_ERR_INVALID;
     }
     if (n == 0) return BSMPS_OK;
-    if (!pre || !lambda_prev || !gamma || !T) {
-        bsmps_set_error("bsmps_site_contract: null pointer");
+    if (!pre || !lambda_prev || !gamma || !T || (om && !om->map)) {
+        bsmps_set_error("%s: null pointer", name);
         return BSMPS_ERR_INVALID;
     }
     const int64_t ncols = (int64_t)chi * d;
     const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    const int64_t gy_gauss = (n + GaussShape<double>::ROWS - 1) / GaussShape<double>::ROWS;
-    if (gx > 0x7fffffff || gy > 65535 || (ctx->contract_gauss[0] && gy_gauss > 65535)) {
-        bsmps_set_error("bsmps_site_contract: problem too large for one launch");
+    const int64_t gy_gauss = (n + GaussShape<E>::ROWS - 1) / GaussShape<E>::ROWS;
+    if (gx > 0x7fffffff || gy > 65535 || (ctx->contract_gauss[form] && gy_gauss > 65535)) {
+        bsmps_set_error("%s: problem too large for one launch", name);
         return BSMPS_ERR_UNSUPPORTED;
     }
-    if (ctx->contract_gauss[0])
-        hipLaunchKernelGGL(site_gemm3m_kernel<double>, gauss_grid<double>(n, ncols), dim3(256), 0, ctx->stream, n,
-                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
-    else
-        hipLaunchKernelGGL(site_gemm_kernel<double>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
-                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
+    const ColMap none{0, nullptr};
+    const dim3 bgrid((unsigned)gx, (unsigned)gy);
+    if (ctx->contract_gauss[form]) {
+        if (om)
+            hipLaunchKernelGGL((site_gemm3m_kernel<E, true>), gauss_grid<E>(n, ncols), dim3(256), 0, ctx->stream, n,
+                               chi_prev, (int)ncols, pre, lambda_prev, gamma, T, *om);
+        else
+            hipLaunchKernelGGL(site_gemm3m_kernel<E>, gauss_grid<E>(n, ncols), dim3(256), 0, ctx->stream, n, chi_prev,
+                               (int)ncols, pre, lambda_prev, gamma, T, none);
+    } else {
+        if (om)
+            hipLaunchKernelGGL((site_gemm_kernel<E, true>), bgrid, dim3(256), 0, ctx->stream, n, chi_prev, (int)ncols,
+                               pre, lambda_prev, gamma, T, *om);
+        else
+            hipLaunchKernelGGL(site_gemm_kernel<E>, bgrid, dim3(256), 0, ctx->stream, n, chi_prev, (int)ncols, pre,
+                               lambda_prev, gamma, T, none);
+    }
     BSMPS_HIP(hipGetLastError());
     return BSMPS_OK;
+}
+
+extern "C" int bsmps_site_contract(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const double* pre,
+                                   const double* lambda_prev, const double* gamma, double* T) {
+    return site_contract_mfma<double>(ctx, "bsmps_site_contract", n, chi_prev, chi, d, pre, lambda_prev, gamma, T,
+                                      nullptr);
+}
+
+extern "C" int bsmps_site_contract_cols(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const double* pre,
+                                        const double* lambda_prev, const double* gamma, double* T, int64_t ldT,
+                                        const int32_t* cmap) {
+    const ColMap om{ldT, cmap};
+    return site_contract_mfma<double>(ctx, "bsmps_site_contract_cols", n, chi_prev, chi, d, pre, lambda_prev, gamma,
+                                      T, &om);
 }
 
 extern "C" int bsmps_sample_site(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const double* T,
@@ -1683,30 +1772,16 @@ extern "C" int bsmps_philox(bsmps_ctx* ctx, uint64_t seed, uint32_t stream0, int
 
 extern "C" int bsmps_site_contract_c64(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const float* pre,
                                        const double* lambda_prev, const float* gamma, float* T) {
-    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
-        bsmps_set_error("bsmps_site_contract_c64: invalid arguments");
-        return BSMPS_ERR_INVALID;
-    }
-    if (n == 0) return BSMPS_OK;
-    if (!pre || !lambda_prev || !gamma || !T) {
-        bsmps_set_error("bsmps_site_contract_c64: null pointer");
-        return BSMPS_ERR_INVALID;
-    }
-    const int64_t ncols = (int64_t)chi * d;
-    const int64_t gx = (2 * ncols + TN - 1) / TN, gy = (n + TM - 1) / TM;
-    const int64_t gy_gauss = (n + GaussShape<float>::ROWS - 1) / GaussShape<float>::ROWS;
-    if (gx > 0x7fffffff || gy > 65535 || (ctx->contract_gauss[1] && gy_gauss > 65535)) {
-        bsmps_set_error("bsmps_site_contract_c64: problem too large for one launch");
-        return BSMPS_ERR_UNSUPPORTED;
-    }
-    if (ctx->contract_gauss[1])
-        hipLaunchKernelGGL(site_gemm3m_kernel<float>, gauss_grid<float>(n, ncols), dim3(256), 0, ctx->stream, n,
-                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
-    else
-        hipLaunchKernelGGL(site_gemm_kernel<float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, n,
-                           chi_prev, (int)ncols, pre, lambda_prev, gamma, T);
-    BSMPS_HIP(hipGetLastError());
-    return BSMPS_OK;
+    return site_contract_mfma<float>(ctx, "bsmps_site_contract_c64", n, chi_prev, chi, d, pre, lambda_prev, gamma, T,
+                                     nullptr);
+}
+
+extern "C" int bsmps_site_contract_c64_cols(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d, const float* pre,
+                                            const double* lambda_prev, const float* gamma, float* T, int64_t ldT,
+                                            const int32_t* cmap) {
+    const ColMap om{ldT, cmap};
+    return site_contract_mfma<float>(ctx, "bsmps_site_contract_c64_cols", n, chi_prev, chi, d, pre, lambda_prev,
+                                     gamma, T, &om);
 }
 
 extern "C" int bsmps_sample_site_c64(bsmps_ctx* ctx, int64_t n, int chi, int d, int dd, const float* T,
@@ -1850,11 +1925,13 @@ extern "C" int bsmps_split_rows_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, 
 }
 
 // The split-form contraction of either format.  Long k runs in segments of kSplitKSeg values
-// with the partials in the context workspace and an FP64 combine in segment order.
+// with the partials in the context workspace and an FP64 combine in segment order.  With a column
+// map (om non-null) the unsegmented product is stored through it and a segmented one is combined
+// through it; the partials keep their own contiguous rows.
 template <class F>
 int launch_split_contract(bsmps_ctx* ctx, int64_t n, int64_t ncols, int64_t kpad, int64_t ncols_pad, int64_t gx,
                           int64_t gy, const typename F::E* apl, const float* rowscale, const typename F::E* bpl,
-                          const float* colscale, float* T) {
+                          const float* colscale, float* T, const ColMap* om) {
     const int segs = kpad > kSplitKMin ? (int)((kpad + kSplitKSeg - 1) / kSplitKSeg) : 1;
     float* out = T;
     const int64_t count = n * ncols * 2;
@@ -1865,41 +1942,69 @@ int launch_split_contract(bsmps_ctx* ctx, int64_t n, int64_t ncols, int64_t kpad
     }
     const dim3 grid((unsigned)gemm_tile_grid(gx, gy), (unsigned)segs);
     const int kseg = segs > 1 ? (int)kSplitKSeg : (int)kpad;
-    if constexpr (F::DMA)
-        hipLaunchKernelGGL(site_gemm_split_dma_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
-                           (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg);
-    else
-        hipLaunchKernelGGL(site_gemm_split_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
-                           (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg);
+    const ColMap none{0, nullptr};
+    if (om && segs == 1) {
+        if constexpr (F::DMA)
+            hipLaunchKernelGGL((site_gemm_split_dma_kernel<F, true>), grid, dim3(256), 0, ctx->stream, n, (int)ncols,
+                               (int)kpad, (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg, *om);
+        else
+            hipLaunchKernelGGL(site_gemm_split_cols_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols,
+                               (int)kpad, (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg, *om);
+    } else {
+        if constexpr (F::DMA)
+            hipLaunchKernelGGL(site_gemm_split_dma_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
+                               (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg, none);
+        else
+            hipLaunchKernelGGL(site_gemm_split_kernel<F>, grid, dim3(256), 0, ctx->stream, n, (int)ncols, (int)kpad,
+                               (int)ncols_pad, apl, rowscale, bpl, colscale, out, kseg);
+    }
     BSMPS_HIP(hipGetLastError());
     if (segs > 1) {
-        hipLaunchKernelGGL(split_k_combine_kernel, dim3((unsigned)std::min<int64_t>((count + 255) / 256, 65536)),
-                           dim3(256), 0, ctx->stream, out, count, segs, T);
+        const dim3 cgrid((unsigned)std::min<int64_t>((count + 255) / 256, 65536));
+        if (om)
+            hipLaunchKernelGGL(split_k_combine_cols_kernel, cgrid, dim3(256), 0, ctx->stream, out, n, (int)ncols, segs,
+                               T, *om);
+        else
+            hipLaunchKernelGGL(split_k_combine_kernel, cgrid, dim3(256), 0, ctx->stream, out, count, segs, T);
         BSMPS_HIP(hipGetLastError());
     }
     return BSMPS_OK;
 }
 
-extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
-                                          const void* pre_planes, const void* gamma_planes, float* T) {
-    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
-        bsmps_set_error("bsmps_site_contract_bf16x6: invalid arguments");
+static int site_contract_bf16x6(bsmps_ctx* ctx, const char* name, int64_t n, int chi_prev, int chi, int d,
+                                const void* pre_planes, const void* gamma_planes, float* T, const ColMap* om) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0 || (om && om->ld <= 0)) {
+        bsmps_set_error("%s: invalid arguments", name);
         return BSMPS_ERR_INVALID;
     }
     if (n == 0) return BSMPS_OK;
-    if (!pre_planes || !gamma_planes || !T) {
-        bsmps_set_error("bsmps_site_contract_bf16x6: null pointer");
+    if (!pre_planes || !gamma_planes || !T || (om && !om->map)) {
+        bsmps_set_error("%s: null pointer", name);
         return BSMPS_ERR_INVALID;
     }
     const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
     const int64_t gx = ncols_pad / X6_TN, gy = (n + X6_TM - 1) / X6_TM;
     if (gy > 0x7fffffff || ncols > 0x7fffffff || gemm_tile_grid(gx, gy) > ((int64_t)1 << 32) / 256 - 1) {
-        bsmps_set_error("bsmps_site_contract_bf16x6: problem too large for one launch");
+        bsmps_set_error("%s: problem too large for one launch", name);
         return BSMPS_ERR_UNSUPPORTED;
     }
     return launch_split_contract<Bf16x6Fmt>(ctx, n, ncols, kpad, ncols_pad, gx, gy,
                                             reinterpret_cast<const __bf16*>(pre_planes), nullptr,
-                                            reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T);
+                                            reinterpret_cast<const __bf16*>(gamma_planes), nullptr, T, om);
+}
+
+extern "C" int bsmps_site_contract_bf16x6(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                          const void* pre_planes, const void* gamma_planes, float* T) {
+    return site_contract_bf16x6(ctx, "bsmps_site_contract_bf16x6", n, chi_prev, chi, d, pre_planes, gamma_planes, T,
+                                nullptr);
+}
+
+extern "C" int bsmps_site_contract_bf16x6_cols(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                               const void* pre_planes, const void* gamma_planes, float* T,
+                                               int64_t ldT, const int32_t* cmap) {
+    const ColMap om{ldT, cmap};
+    return site_contract_bf16x6(ctx, "bsmps_site_contract_bf16x6_cols", n, chi_prev, chi, d, pre_planes, gamma_planes,
+                                T, &om);
 }
 
 extern "C" int64_t bsmps_fp16x3_plane_bytes(int chi_prev, int chi, int d) {
@@ -1965,25 +2070,41 @@ extern "C" int bsmps_split_rows_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, 
     return BSMPS_OK;
 }
 
-extern "C" int bsmps_site_contract_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
-                                          const void* pre_planes, const float* rowscale, const void* gamma_planes,
-                                          const float* colscale, float* T) {
-    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0) {
-        bsmps_set_error("bsmps_site_contract_fp16x3: invalid arguments");
+static int site_contract_fp16x3(bsmps_ctx* ctx, const char* name, int64_t n, int chi_prev, int chi, int d,
+                                const void* pre_planes, const float* rowscale, const void* gamma_planes,
+                                const float* colscale, float* T, const ColMap* om) {
+    if (!ctx || n < 0 || chi_prev <= 0 || chi <= 0 || d <= 0 || (om && om->ld <= 0)) {
+        bsmps_set_error("%s: invalid arguments", name);
         return BSMPS_ERR_INVALID;
     }
     if (n == 0) return BSMPS_OK;
-    if (!pre_planes || !rowscale || !gamma_planes || !colscale || !T) {
-        bsmps_set_error("bsmps_site_contract_fp16x3: null pointer");
+    if (!pre_planes || !rowscale || !gamma_planes || !colscale || !T || (om && !om->map)) {
+        bsmps_set_error("%s: null pointer", name);
         return BSMPS_ERR_INVALID;
     }
     const int64_t ncols = (int64_t)chi * d, kpad = x6_kpad(chi_prev), ncols_pad = x6_cols(ncols);
     const int64_t gx = ncols_pad / X6_TN, gy = (n + X6_TM - 1) / X6_TM;
     if (gy > 0x7fffffff || ncols > 0x7fffffff || gemm_tile_grid(gx, gy) > ((int64_t)1 << 32) / 256 - 1) {
-        bsmps_set_error("bsmps_site_contract_fp16x3: problem too large for one launch");
+        bsmps_set_error("%s: problem too large for one launch", name);
         return BSMPS_ERR_UNSUPPORTED;
     }
     return launch_split_contract<Fp16x3Fmt>(ctx, n, ncols, kpad, ncols_pad, gx, gy,
                                             reinterpret_cast<const _Float16*>(pre_planes), rowscale,
-                                            reinterpret_cast<const _Float16*>(gamma_planes), colscale, T);
+                                            reinterpret_cast<const _Float16*>(gamma_planes), colscale, T, om);
+}
+
+extern "C" int bsmps_site_contract_fp16x3(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                          const void* pre_planes, const float* rowscale, const void* gamma_planes,
+                                          const float* colscale, float* T) {
+    return site_contract_fp16x3(ctx, "bsmps_site_contract_fp16x3", n, chi_prev, chi, d, pre_planes, rowscale,
+                                gamma_planes, colscale, T, nullptr);
+}
+
+extern "C" int bsmps_site_contract_fp16x3_cols(bsmps_ctx* ctx, int64_t n, int chi_prev, int chi, int d,
+                                               const void* pre_planes, const float* rowscale,
+                                               const void* gamma_planes, const float* colscale, float* T,
+                                               int64_t ldT, const int32_t* cmap) {
+    const ColMap om{ldT, cmap};
+    return site_contract_fp16x3(ctx, "bsmps_site_contract_fp16x3_cols", n, chi_prev, chi, d, pre_planes, rowscale,
+                                gamma_planes, colscale, T, &om);
 }

@@ -190,9 +190,10 @@ class _SiteForm:
 
 class _SiteBlock:
     """One parity block of a site: its left labels ``ia`` and columns ``ic`` (index tensors), the
-    block of Gamma as a [len(ia), len(ic), 1] tensor, Lambda_{i-1} of its labels and, for the
-    split forms, its operand planes."""
-    __slots__ = ("ia", "ic", "rows", "cols", "gamma", "lam", "split")
+    columns again as the device int32 map the contraction stores through (``cmap``), the block of
+    Gamma as a [len(ia), len(ic), 1] tensor, Lambda_{i-1} of its labels and, for the split forms,
+    its operand planes."""
+    __slots__ = ("ia", "ic", "cmap", "rows", "cols", "gamma", "lam", "split")
 
 
 def _split_operand(ctx, precision, gamma, rows, cols, d, lam_prev):
@@ -248,6 +249,7 @@ def _prepare_site(ctx, precision, i, g, rows, cols, lam_prev):
     for ia, ic in pb:
         b = _SiteBlock()
         b.ia, b.ic, b.rows, b.cols = ia, ic, len(ia), len(ic)
+        b.cmap = ic.to(torch.int32).contiguous()      # ic < cols * d, the row length of T
         b.gamma = flat.index_select(0, ia).index_select(1, ic).contiguous().view(b.rows, b.cols, 1)
         b.lam = lam_prev.index_select(0, ia).contiguous()
         b.split = _split_operand(ctx, precision, b.gamma, b.rows, b.cols, 1, b.lam)
@@ -256,39 +258,56 @@ def _prepare_site(ctx, precision, i, g, rows, cols, lam_prev):
     return f
 
 
-def contract_site(ctx, precision, f, nc, pre, lam_prev, T, work):
+def contract_site(ctx, precision, f, nc, pre, lam_prev, T, work, scatter=False):
     """T [nc][cols * d] = (pre [nc][rows] Lambda_{i-1}) Gamma_i for the site form f (every sample's
     carried vector against the site tensor), in one product or, when f splits by parity, in two
-    half-size ones: each block's labels gathered from the carried vectors, its product into
-    ``work.T`` and its columns scattered into T.  ``work`` holds the reduced forms' row planes
+    half-size ones: each block's labels gathered from the carried vectors and its product stored
+    straight into its columns of T through the block's column map (the ``_cols`` entry points).
+    BF16x3, and any form with ``scatter``, instead writes the block's product to ``work.T`` and
+    scatters its columns into T (the same values).  ``work`` holds the reduced forms' row planes
     (``pre_planes``, ``rowscale``) and the block output buffer ``T`` (at least nc * cols * d)."""
     if f.blocks is None:
         _contract(ctx, precision, nc, f.rows, f.cols, f.d, runtime.ptr(pre), lam_prev, f.gamma, f.split, T, work)
         return
-    Tv = T[:nc * f.cols * f.d].view(nc, f.cols * f.d)
+    ld = f.cols * f.d
+    Tv = T[:nc * ld].view(nc, ld)
     pre_v = pre[:nc * f.rows].view(nc, f.rows)
+    mapped = precision in _MAPPED and not scatter
     for b in f.blocks:
         pre_b = pre_v.index_select(1, b.ia).contiguous()
-        Tb = work.T[:nc * b.cols]
-        _contract(ctx, precision, nc, b.rows, b.cols, 1, runtime.ptr(pre_b), b.lam, b.gamma, b.split, Tb, work)
-        Tv.index_copy_(1, b.ic, Tb.view(nc, b.cols))
+        if mapped:
+            _contract(ctx, precision, nc, b.rows, b.cols, 1, runtime.ptr(pre_b), b.lam, b.gamma, b.split, T, work,
+                      out_cols=(ld, b.cmap))
+        else:
+            Tb = work.T[:nc * b.cols]
+            _contract(ctx, precision, nc, b.rows, b.cols, 1, runtime.ptr(pre_b), b.lam, b.gamma, b.split, Tb, work)
+            Tv.index_copy_(1, b.ic, Tb.view(nc, b.cols))
 
 
-def _contract(ctx, precision, nc, rows, cols, d, pre_in, lam_prev, gamma, split, T, work):
+_MAPPED = ("fp64", "fp32", "fp16x3", "bf16x6")   # forms with a column-mapped entry point
+
+
+def _contract(ctx, precision, nc, rows, cols, d, pre_in, lam_prev, gamma, split, T, work, out_cols=None):
+    """One contraction call; ``out_cols=(ldT, cmap)`` stores its columns through cmap into rows
+    of ldT (the ``_cols`` entry points)."""
     T = runtime.ptr(T)
+    tail, sfx = ((out_cols[0], runtime.ptr(out_cols[1])), "_cols") if out_cols is not None else ((), "")
     if precision == "fp64":
-        ctx.call("bsmps_site_contract", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(gamma), T)
+        ctx.call("bsmps_site_contract" + sfx, nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(gamma), T,
+                 *tail)
     elif precision == "fp32":
-        ctx.call("bsmps_site_contract_c64", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(gamma), T)
+        ctx.call("bsmps_site_contract_c64" + sfx, nc, rows, cols, d, pre_in, runtime.ptr(lam_prev),
+                 runtime.ptr(gamma), T, *tail)
     elif precision == "fp16x3":
         planes, colscale = split
         ctx.call("bsmps_split_rows_fp16x3", nc, rows, pre_in, runtime.ptr(work.pre_planes),
                  runtime.ptr(work.rowscale))
-        ctx.call("bsmps_site_contract_fp16x3", nc, rows, cols, d, runtime.ptr(work.pre_planes),
-                 runtime.ptr(work.rowscale), runtime.ptr(planes), runtime.ptr(colscale), T)
+        ctx.call("bsmps_site_contract_fp16x3" + sfx, nc, rows, cols, d, runtime.ptr(work.pre_planes),
+                 runtime.ptr(work.rowscale), runtime.ptr(planes), runtime.ptr(colscale), T, *tail)
     elif precision == "bf16x6":
         ctx.call("bsmps_split_rows_bf16x6", nc, rows, pre_in, runtime.ptr(work.pre_planes))
-        ctx.call("bsmps_site_contract_bf16x6", nc, rows, cols, d, runtime.ptr(work.pre_planes), runtime.ptr(split), T)
+        ctx.call("bsmps_site_contract_bf16x6" + sfx, nc, rows, cols, d, runtime.ptr(work.pre_planes),
+                 runtime.ptr(split), T, *tail)
     else:
         hi, lo = split
         ctx.call("bsmps_site_contract_bf16x3", nc, rows, cols, d, pre_in, runtime.ptr(lam_prev), runtime.ptr(hi),

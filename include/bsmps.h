@@ -289,6 +289,26 @@ int bsmps_split_rows_fp16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, const float
 int bsmps_site_contract_fp16x3(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const void *pre_planes,
                                const float *rowscale, const void *gamma_planes, const float *colscale, float *T);
 
+/* The four contractions above with the output stored through a column map (no reference
+ * counterpart; the photon-number parity split of the contraction of sampling_cpu.py:75-80):
+ * output column c < chi*d of sample row s goes to T[s * ldT + cmap[c]] (complex elements, rows of
+ * ldT), so the product of one parity block of a site -- its labels' carried vectors against its
+ * columns -- lands in the site's full T without a separate scatter.  cmap is device int32
+ * [chi*d]; entries outside [0, ldT) are not written, and T's other columns are left as they are.
+ * Everything else (operands, forms, segments, results bit for bit) is as in the plain call. */
+int bsmps_site_contract_cols(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const double *pre,
+                             const double *lambda_prev, const double *gamma, double *T, int64_t ldT,
+                             const int32_t *cmap);
+int bsmps_site_contract_c64_cols(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d, const float *pre,
+                                 const double *lambda_prev, const float *gamma, float *T, int64_t ldT,
+                                 const int32_t *cmap);
+int bsmps_site_contract_bf16x6_cols(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d,
+                                    const void *pre_planes, const void *gamma_planes, float *T, int64_t ldT,
+                                    const int32_t *cmap);
+int bsmps_site_contract_fp16x3_cols(bsmps_ctx *ctx, int64_t n, int chi_prev, int chi, int d,
+                                    const void *pre_planes, const float *rowscale, const void *gamma_planes,
+                                    const float *colscale, float *T, int64_t ldT, const int32_t *cmap);
+
 /* Counter-based Philox4x32-10 streams (Salmon et al., SC'11), identical on host and device.
  * out[r][i] (r < n_streams, i < n) is element first+i of stream stream0+r under `seed`:
  * counter (e_lo, e_hi, stream, 0) with e = first+i, key (seed_lo, seed_hi).  A uniform is

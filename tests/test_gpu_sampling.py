@@ -302,12 +302,13 @@ def _parity_gamma(rng, rows, cols, d, parity_a, parity_b):
     return np.where(keep, g, 0)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp16x3"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x6", "fp16x3"])
 def test_parity_split_contraction_matches_full(prec, monkeypatch):
     """Site tensors whose entries vanish by photon-number parity are contracted as two half-size
-    products (gathered labels, scattered columns); the result equals the full product (FP64 within
-    1e-13 of its scale, FP16x3 within its FP32-level bound), tensors without the structure are
-    not split, and a tensor of two disconnected label groups still is."""
+    products (gathered labels, columns stored through the block's column map); the result equals
+    the full product (FP64 within 1e-13 of its scale, the others within their FP32-level bound)
+    and, bit for bit, the same blocks written to a buffer and scattered; tensors without the
+    structure are not split, and a tensor of two disconnected label groups still is."""
     from bosonsupremacy_amd import runtime, sampling
     monkeypatch.setattr(sampling, "PARITY_MIN_ENTRIES", 0)   # a small tensor, split regardless
     ctx = runtime.context()
@@ -331,6 +332,9 @@ def test_parity_split_contraction_matches_full(prec, monkeypatch):
     pre_d = torch.from_numpy(pre).to(device=ctx.device, dtype=ctype).reshape(-1).contiguous()
     T = torch.empty((n * cols * d,), dtype=ctype, device=ctx.device)
     sampling.contract_site(ctx, prec, f, n, pre_d, lam_d, T, work)
+    T2 = torch.full((n * cols * d,), 7.0, dtype=ctype, device=ctx.device)
+    sampling.contract_site(ctx, prec, f, n, pre_d, lam_d, T2, work, scatter=True)
+    assert torch.equal(T, T2)
     got = runtime.to_host(T).reshape(n, cols * d).astype(np.complex128)
     if prec == "fp64":
         assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
@@ -351,6 +355,42 @@ def test_parity_split_contraction_matches_full(prec, monkeypatch):
     flat = G3.reshape(rows, cols * d)
     (ia0, ic0), (ia1, ic1) = [(a.cpu().numpy(), c.cpu().numpy()) for a, c in blocks]
     assert not flat[np.ix_(ia0, ic1)].any() and not flat[np.ix_(ia1, ic0)].any()
+
+
+@pytest.mark.parametrize("prec,chi_p", [("fp64", 300), ("fp32", 300), ("bf16x6", 300), ("fp16x3", 300),
+                                         ("bf16x6", 4500), ("fp16x3", 4500)])
+def test_column_mapped_contractions_match_plain(prec, chi_p, contract_form):
+    """The _cols entry points store output column c at T[s * ldT + cmap[c]]: every mapped column
+    equals the plain call's column bit for bit, entries of cmap outside [0, ldT) are not written,
+    and T's unmapped columns keep what they held -- in both FP64 / FP32 forms and, at chi_prev =
+    4500, through the reduced forms' k segments and their combine."""
+    from bosonsupremacy_amd import runtime, sampling
+    ctx = runtime.context()
+    rng = np.random.default_rng(chi_p + len(prec))
+    n, chi, d, ld = 200, 37, 3, 150
+    ctype = torch.complex128 if prec == "fp64" else torch.complex64
+    print(f"column map {prec} chi_prev={chi_p} n={n} cols={chi * d} ld={ld} form={contract_form}", flush=True)
+    G = rng.normal(size=(chi_p, chi, d)) + 1j * rng.normal(size=(chi_p, chi, d))
+    lam = rng.uniform(0.1, 1, size=chi_p)
+    pre = rng.normal(size=(n, chi_p)) + 1j * rng.normal(size=(n, chi_p))
+    G_d = torch.from_numpy(G).to(device=ctx.device, dtype=ctype).contiguous()
+    pre_d = torch.from_numpy(pre).to(device=ctx.device, dtype=ctype).reshape(-1).contiguous()
+    lam_d = runtime.to_device(lam)
+    split = sampling._split_operand(ctx, prec, G_d, chi_p, chi, d, lam_d)
+    work = sampling._Work(ctx, prec, n, chi_p, chi * d, ctype)
+    plain = torch.empty((n * chi * d,), dtype=ctype, device=ctx.device)
+    sampling._contract(ctx, prec, n, chi_p, chi, d, runtime.ptr(pre_d), lam_d, G_d, split, plain, work)
+    cmap = rng.permutation(ld)[:chi * d].astype(np.int32)
+    cmap[5], cmap[40] = -1, ld                      # not written
+    Tm = torch.full((n * ld,), 3.0 - 1.0j, dtype=ctype, device=ctx.device)
+    sampling._contract(ctx, prec, n, chi_p, chi, d, runtime.ptr(pre_d), lam_d, G_d, split, Tm, work,
+                       out_cols=(ld, torch.from_numpy(cmap).to(ctx.device)))
+    got, want = runtime.to_host(Tm).reshape(n, ld), runtime.to_host(plain).reshape(n, chi * d)
+    ok = (cmap >= 0) & (cmap < ld)
+    assert np.array_equal(np.ascontiguousarray(got[:, cmap[ok]]).view(np.uint8),
+                          np.ascontiguousarray(want[:, ok]).view(np.uint8))
+    untouched = np.setdiff1d(np.arange(ld), cmap[ok])
+    assert len(untouched) > 0 and (got[:, untouched] == 3.0 - 1.0j).all()
 
 
 def test_split_gamma_planes_and_column_scales_exact():
