@@ -42,3 +42,18 @@ def test_error_paths_without_device():
     assert lib.bsmps_set_contract_form(None, 0, 1) == _lib.BSMPS_ERR_INVALID
     with pytest.raises(_lib.BsmpsError):
         _lib.check(lib.bsmps_synchronize(None))
+
+
+def test_column_mapped_contractions_refuse_bad_arguments_without_device():
+    """The _cols contractions refuse a call without a context before any device work and name
+    themselves in the error (their other refusals need a context: tests/test_gpu_sampling.py)."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libbsmps.so not built")
+    lib = _lib.load()
+    p = ctypes.c_void_p(8)          # never dereferenced: every call below is refused first
+    for name, ops in (("bsmps_site_contract_cols", (p, p, p, p)), ("bsmps_site_contract_c64_cols", (p, p, p, p)),
+                      ("bsmps_site_contract_bf16x6_cols", (p, p, p)),
+                      ("bsmps_site_contract_fp16x3_cols", (p, p, p, p, p))):
+        fn = getattr(lib, name)
+        assert fn(None, 10, 4, 3, 2, *ops, 6, p) == _lib.BSMPS_ERR_INVALID
+        assert name.encode() in lib.bsmps_last_error()

@@ -393,6 +393,23 @@ def test_column_mapped_contractions_match_plain(prec, chi_p, contract_form):
     assert len(untouched) > 0 and (got[:, untouched] == 3.0 - 1.0j).all()
 
 
+def test_column_mapped_contractions_refuse_bad_maps():
+    """A non-positive row length ldT or a missing column map is refused before any launch, with
+    the entry point named; T is left untouched."""
+    from bosonsupremacy_amd import _lib, runtime
+    ctx = runtime.context()
+    T = torch.full((8,), 5.0, dtype=torch.float64, device=ctx.device)
+    cmap = torch.zeros((4,), dtype=torch.int32, device=ctx.device)
+    x = runtime.ptr(T)
+    for ld, cm in ((0, runtime.ptr(cmap)), (-3, runtime.ptr(cmap)), (8, None)):
+        st = ctx.lib.bsmps_site_contract_cols(ctx.handle, 2, 2, 2, 2, x, x, x, x, ld, cm)
+        assert st == _lib.BSMPS_ERR_INVALID and b"bsmps_site_contract_cols" in ctx.lib.bsmps_last_error()
+        st = ctx.lib.bsmps_site_contract_fp16x3_cols(ctx.handle, 2, 2, 2, 2, x, x, x, x, x, ld, cm)
+        assert st == _lib.BSMPS_ERR_INVALID and b"fp16x3_cols" in ctx.lib.bsmps_last_error()
+    torch.cuda.synchronize()
+    assert (T == 5.0).all()
+
+
 def test_split_gamma_planes_and_column_scales_exact():
     """The Gamma planes of both split forms, entry by entry, against a NumPy restatement of the
     split, on a ragged shape (k and columns padded) with all-zero columns: BF16x6 parts add up to
